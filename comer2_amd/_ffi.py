@@ -1,0 +1,96 @@
+"""ctypes view of the C ABI declared in include/comer2_hip.h."""
+import ctypes as C
+import os
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_HIP = os.path.join(_HERE, "libcomer2_hip.so")
+LIB_SYNTH = os.path.join(_HERE, "libc2synth.so")
+
+
+class Profiles(C.Structure):
+    _fields_ = [("nprof", C.c_int32), ("npos", C.c_int64),
+                ("bkg", C.POINTER(C.c_float)), ("eno", C.POINTER(C.c_float)),
+                ("len", C.POINTER(C.c_int32)), ("dist", C.POINTER(C.c_uint32)),
+                ("trgfrq", C.POINTER(C.c_float)), ("ssprb", C.POINTER(C.c_float)),
+                ("aa", C.POINTER(C.c_char)), ("sss", C.POINTER(C.c_char)),
+                ("trans", C.POINTER(C.c_float))]
+
+
+class Options(C.Structure):
+    _fields_ = [("ssswgt", C.c_float), ("mapaln", C.c_int32), ("minpp", C.c_float),
+                ("evalue", C.c_float), ("ssemodel", C.c_int32), ("xuninf", C.c_int32),
+                ("ung_lambda", C.c_float), ("ung_K", C.c_float), ("ung_H", C.c_float),
+                ("gap_lambda", C.c_float), ("gap_K", C.c_float), ("gap_alpha", C.c_float),
+                ("gap_beta", C.c_float), ("max_map_pairs", C.c_int32)]
+
+
+class Results(C.Structure):
+    _fields_ = [("npassed", C.c_int32), ("nposits", C.c_int64), ("maxlen", C.c_int32),
+                ("nmap", C.c_int32), ("alndata", C.POINTER(C.c_float)),
+                ("order", C.POINTER(C.c_int32)), ("alns", C.POINTER(C.c_char)),
+                ("alnstride", C.c_int64), ("scorethld", C.c_float), ("logevthld", C.c_float),
+                ("search_space", C.c_double)]
+
+
+NFIELDS = 18
+(F_SCORE, F_ORGPRONO, F_PRONEWDST, F_PSTVS, F_BEGCOORDS, F_ENDCOORDS,
+ F_LMBDEST, F_KEST, F_EVALUE, F_EPA, F_MIN, F_MAX, F_E, F_H, F_LMBD, F_K,
+ F_IDNTS, F_NGAPS) = range(18)
+
+
+def options_struct(d):
+    o = Options()
+    for k, _ in Options._fields_:
+        setattr(o, k, d[k])
+    return o
+
+
+def load_synth():
+    lib = C.CDLL(LIB_SYNTH)
+    lib.c2s_generate.restype = C.c_void_p
+    lib.c2s_generate.argtypes = [C.c_uint64, C.c_int32, C.c_double, C.c_double,
+                                 C.c_int32, C.c_int32, C.c_int32, C.c_double]
+    lib.c2s_view.restype = C.POINTER(Profiles)
+    lib.c2s_view.argtypes = [C.c_void_p]
+    lib.c2s_orgidx.restype = C.POINTER(C.c_int32)
+    lib.c2s_orgidx.argtypes = [C.c_void_p]
+    lib.c2s_free.argtypes = [C.c_void_p]
+    lib.c2s_subset.restype = C.c_void_p
+    lib.c2s_subset.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    return lib
+
+
+class SynthDB:
+    """Owned synthetic profile set (see c2_synth.cpp for the distributions)."""
+    _lib = None
+
+    def __init__(self, seed, nprof, median, sigma=0.668, minlen=30, maxlen=800,
+                 sort_desc=True, conc=20.0, _handle=None):
+        if SynthDB._lib is None:
+            SynthDB._lib = load_synth()
+        self.h = _handle if _handle is not None else SynthDB._lib.c2s_generate(
+            seed, nprof, median, sigma, minlen, maxlen, 1 if sort_desc else 0, conc)
+        self.view = SynthDB._lib.c2s_view(self.h).contents
+
+    def subset(self, p0, n):
+        return SynthDB(0, 0, 0, _handle=SynthDB._lib.c2s_subset(self.h, p0, n))
+
+    @property
+    def nprof(self):
+        return self.view.nprof
+
+    @property
+    def npos(self):
+        return self.view.npos
+
+    def lengths(self):
+        return np.ctypeslib.as_array(self.view.len, (self.nprof,)).copy()
+
+    def dists(self):
+        return np.ctypeslib.as_array(self.view.dist, (self.nprof,)).copy()
+
+    def __del__(self):
+        if getattr(self, "h", None) and SynthDB._lib is not None:
+            SynthDB._lib.c2s_free(self.h)
+            self.h = None

@@ -1,0 +1,49 @@
+"""Builds the gfx950 HIP library and the host-only generator library in-tree.
+
+    python -m comer2_amd.build        (or comer2_amd.build.build())
+"""
+import os
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+CSRC = os.path.join(HERE, "csrc")
+ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
+HIPCC = os.path.join(ROCM, "bin", "hipcc")
+
+HIP_SOURCES = ["c2_phase1.hip", "c2_lib.cpp", "c2_prep.cpp"]
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+             "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+             "-Wno-unused-command-line-argument"]
+
+
+def _run(cmd):
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        sys.stderr.write(r.stdout + r.stderr)
+        raise RuntimeError("build failed: " + " ".join(cmd[:3]) + " ...")
+    return r
+
+
+def build_synth():
+    out = os.path.join(HERE, "libc2synth.so")
+    _run(["g++", "-O2", "-fopenmp", "-fPIC", "-shared", "-ffp-contract=off", "-o", out,
+          os.path.join(CSRC, "c2_synth.cpp")])
+    return out
+
+
+def build_hip():
+    out = os.path.join(HERE, "libcomer2_hip.so")
+    tmp = out + ".tmp"
+    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
+    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + srcs)
+    os.replace(tmp, out)
+    return out
+
+
+def build():
+    return build_synth(), build_hip()
+
+
+if __name__ == "__main__":
+    print(build())

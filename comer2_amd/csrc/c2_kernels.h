@@ -1,0 +1,39 @@
+/*
+ * c2_kernels.h -- launch interfaces of the gfx950 kernels (internal to
+ * libcomer2_hip.so; the public boundary is include/comer2_hip.h).
+ */
+#ifndef C2_KERNELS_H
+#define C2_KERNELS_H
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+
+struct P1Args {
+    /* DB chunk (device SoA) */
+    const float* trgfrq; const float* ssprb; const float* trans; const float* bkg;
+    const int32_t* len; const uint32_t* dist; const int32_t* addrpro;
+    int64_t npos; int32_t nprof;
+    /* query preparation, row-padded to Lpad = nstripes*64*R */
+    const float* qv;      /* [20][Lpad] target frequencies */
+    const float* qtr;     /* [7][Lpad]  MMp MIc MDp IMp IIc DMp DDp */
+    const float* QT;      /* [30][Lpad] per-row SS table */
+    const float* qh;      /* [Lpad]     p_H + 0.1 */
+    const float* lleft;   /* [5][Lpad]  column -1 left boundary */
+    const float* ldiag;   /* [5][Lpad]  column -1 diagonal boundary */
+    const float* qbkg;    /* [20] query background */
+    int32_t Lq, Lpad, nstripes;
+    float shift, ssswgt;
+    /* segments of whole profiles */
+    const int32_t* seg_p; /* [nseg+1] */
+    int32_t nseg;
+    /* outputs */
+    unsigned long long* keys;   /* [nprof] */
+    float* bot;                 /* [5][npos] stripe hand-off (nstripes > 1) */
+    uint32_t* btck;             /* packed backtrack codes (BTCK launches) */
+    const int64_t* btck_off;    /* per segment word offset */
+};
+
+size_t c2_phase1_lds_bytes(int R, int NW, int nstripes);
+hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream_t stream);
+
+#endif

@@ -1,0 +1,319 @@
+/*
+ * c2_lib.cpp -- implementation of the C ABI (include/comer2_hip.h).
+ */
+#include <cstring>
+#include <cmath>
+#include <cstdio>
+#include <algorithm>
+#include "c2_lib.h"
+
+#define HIPCHK(call)                                                             \
+    do {                                                                         \
+        hipError_t e_ = (call);                                                  \
+        if (e_ != hipSuccess) {                                                  \
+            h->err = std::string(#call) + ": " + hipGetErrorString(e_);          \
+            return CBP_ERR_HIP;                                                  \
+        }                                                                        \
+    } while (0)
+
+static int fail(cbp_handle* h, int code, const std::string& msg) {
+    if (h) h->err = msg;
+    return code;
+}
+
+void HostSet::copy_from(const cbp_profiles* s) {
+    const size_t np = (size_t)s->nprof, ns = (size_t)s->npos;
+    bkg.assign(s->bkg, s->bkg + 20 * np);
+    eno.assign(s->eno, s->eno + np);
+    len.assign(s->len, s->len + np);
+    dist.assign(s->dist, s->dist + np);
+    trgfrq.assign(s->trgfrq, s->trgfrq + 20 * ns);
+    ssprb.assign(s->ssprb, s->ssprb + 3 * ns);
+    aa.assign(s->aa, s->aa + ns);
+    sss.assign(s->sss, s->sss + ns);
+    trans.assign(s->trans, s->trans + 7 * (ns + np));
+    v.nprof = s->nprof; v.npos = s->npos;
+    v.bkg = bkg.data(); v.eno = eno.data(); v.len = len.data(); v.dist = dist.data();
+    v.trgfrq = trgfrq.data(); v.ssprb = ssprb.data(); v.aa = aa.data(); v.sss = sss.data();
+    v.trans = trans.data();
+}
+
+static int dalloc(cbp_handle* h, DBuf& b, size_t n) {
+    if (b.p && b.n >= n) return CBP_OK;
+    if (b.p) { h->used -= b.n; (void)hipFree(b.p); b.p = nullptr; b.n = 0; }
+    if (n == 0) n = 8;
+    if (h->budget && h->used + n > h->budget)
+        return fail(h, CBP_ERR_MEM, "device memory budget exceeded");
+    HIPCHK(hipMalloc(&b.p, n));
+    b.n = n; h->used += n;
+    return CBP_OK;
+}
+
+template <class T>
+static int dupload(cbp_handle* h, DBuf& b, const T* src, size_t count) {
+    int rc = dalloc(h, b, sizeof(T) * count);
+    if (rc) return rc;
+    if (count) HIPCHK(hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, h->stream));
+    return CBP_OK;
+}
+
+static int check_set(const cbp_profiles* s) {
+    if (!s || s->nprof < 1 || s->npos < 1 || !s->bkg || !s->eno || !s->len || !s->dist ||
+        !s->trgfrq || !s->ssprb || !s->aa || !s->sss || !s->trans)
+        return 0;
+    int64_t acc = 0;
+    for (int32_t p = 0; p < s->nprof; p++) {
+        if (s->len[p] < 1 || s->len[p] > 65535 || (int64_t)s->dist[p] != acc) return 0;
+        acc += s->len[p];
+    }
+    if (acc != s->npos) return 0;
+    /* SS probabilities index the 30-entry SS tables: they must lie in [0,1] */
+    for (size_t i = 0; i < (size_t)3 * s->npos; i++)
+        if (!(s->ssprb[i] >= 0.0f && s->ssprb[i] <= 1.0f)) return 0;
+    return 1;
+}
+
+extern "C" {
+
+int cbp_create(int device, size_t mem_budget, int area, cbp_handle** out)
+{
+    (void)area;
+    if (!out) return CBP_ERR_ARG;
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) return CBP_ERR_NODEV;
+    if (device < 0 || device >= ndev) return CBP_ERR_ARG;
+    cbp_handle* h = new cbp_handle();
+    h->device = device;
+    h->budget = mem_budget;
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreate(&h->stream) != hipSuccess ||
+        hipEventCreate(&h->ev0) != hipSuccess || hipEventCreate(&h->ev1) != hipSuccess) {
+        delete h;
+        return CBP_ERR_HIP;
+    }
+    *out = h;
+    return CBP_OK;
+}
+
+int cbp_destroy(cbp_handle* h)
+{
+    if (!h) return CBP_ERR_ARG;
+    (void)hipSetDevice(h->device);
+    (void)hipStreamSynchronize(h->stream);
+    h->qcache.clear();
+    h->dchunk = DevSet();
+    if (h->ev0) (void)hipEventDestroy(h->ev0);
+    if (h->ev1) (void)hipEventDestroy(h->ev1);
+    if (h->stream) (void)hipStreamDestroy(h->stream);
+    delete h;
+    return CBP_OK;
+}
+
+const char* cbp_last_error(const cbp_handle* h)
+{
+    return h ? h->err.c_str() : "null handle";
+}
+
+int cbp_set_options(cbp_handle* h, const cbp_options* opt)
+{
+    if (!h || !opt) return CBP_ERR_ARG;
+    if (!(opt->ssswgt >= 0.0f) || !(opt->gap_lambda > 0.0f) || !(opt->gap_K > 0.0f))
+        return fail(h, CBP_ERR_ARG, "invalid options");
+    h->opt = *opt;
+    h->have_opt = true;
+    return CBP_OK;
+}
+
+int cbp_upload_tables(cbp_handle* h, const float* ssss900, const float* nnw, int32_t nnw_count)
+{
+    if (!h || !ssss900) return CBP_ERR_ARG;
+    h->ssss.assign(ssss900, ssss900 + 900);
+    if (nnw) {
+        if (nnw_count != 93) return fail(h, CBP_ERR_ARG, "expected 93 SS18 parameters");
+        h->nnw.assign(nnw, nnw + 93);
+    }
+    h->have_tables = true;
+    h->qcache.clear();
+    return CBP_OK;
+}
+
+int cbp_upload_queries(cbp_handle* h, const cbp_profiles* q)
+{
+    if (!h) return CBP_ERR_ARG;
+    if (!check_set(q)) return fail(h, CBP_ERR_ARG, "invalid query profile set");
+    h->queries.copy_from(q);
+    h->have_queries = true;
+    h->qcache.clear();
+    return CBP_OK;
+}
+
+int cbp_upload_chunk(cbp_handle* h, const cbp_profiles* db)
+{
+    if (!h) return CBP_ERR_ARG;
+    if (!check_set(db)) return fail(h, CBP_ERR_ARG, "invalid database profile set");
+    (void)hipSetDevice(h->device);
+    const cbp_profiles& s = *db;
+    const size_t np = s.nprof, ns = s.npos;
+    DevSet& d = h->dchunk;
+    d.nprof = s.nprof; d.npos = s.npos;
+    int rc;
+    if ((rc = dupload(h, d.bkg, s.bkg, 20 * np))) return rc;
+    if ((rc = dupload(h, d.eno, s.eno, np))) return rc;
+    if ((rc = dupload(h, d.len, s.len, np))) return rc;
+    if ((rc = dupload(h, d.dist, s.dist, np))) return rc;
+    if ((rc = dupload(h, d.trgfrq, s.trgfrq, 20 * ns))) return rc;
+    if ((rc = dupload(h, d.ssprb, s.ssprb, 3 * ns))) return rc;
+    if ((rc = dupload(h, d.aa, s.aa, ns))) return rc;
+    if ((rc = dupload(h, d.sss, s.sss, ns))) return rc;
+    if ((rc = dupload(h, d.trans, s.trans, 7 * (ns + np)))) return rc;
+    /* profile index of every position (the reference's pmv2DAddrPro field) */
+    std::vector<int32_t> ap(ns);
+    for (int32_t p = 0; p < s.nprof; p++)
+        std::fill(ap.begin() + s.dist[p], ap.begin() + s.dist[p] + s.len[p], p);
+    if ((rc = dupload(h, d.addrpro, ap.data(), ns))) return rc;
+    /* segments of whole profiles of ~seg_cols columns */
+    h->seg_p.clear();
+    h->seg_p.push_back(0);
+    int64_t acc = 0;
+    for (int32_t p = 0; p < s.nprof; p++) {
+        acc += s.len[p];
+        if (acc >= h->seg_cols) { h->seg_p.push_back(p + 1); acc = 0; }
+    }
+    if (h->seg_p.back() != s.nprof) h->seg_p.push_back(s.nprof);
+    h->nseg = (int32_t)h->seg_p.size() - 1;
+    if ((rc = dupload(h, h->dseg, h->seg_p.data(), h->seg_p.size()))) return rc;
+    if ((rc = dalloc(h, h->keys, sizeof(unsigned long long) * np))) return rc;
+    h->chunk.copy_from(db);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->have_chunk = true;
+    return CBP_OK;
+}
+
+int cbp_set_db_details(cbp_handle* h, uint64_t dblen, uint64_t nseqs)
+{
+    if (!h) return CBP_ERR_ARG;
+    h->dblen = dblen; h->nseqs = nseqs;
+    return CBP_OK;
+}
+
+int cbp_score_threshold(const cbp_options* o, uint64_t qlen, uint64_t dblen,
+                        float* scorethld, float* logevthld)
+{
+    if (!o || !scorethld || !logevthld) return CBP_ERR_ARG;
+    const uint64_t sspace = qlen * dblen;
+    float ev = o->evalue;
+    if (sspace < 1 || o->gap_K <= 0.0f || o->gap_lambda <= 0.0f) return CBP_ERR_ARG;
+    if (ev <= 0.0f) { *scorethld = 999.0f; *logevthld = -99.0f; return CBP_OK; }
+    *logevthld = logf(ev);
+    if (ev < 1.0f) ev *= 10.0f; else ev *= 3.0f;
+    ev /= o->gap_K * (float)sspace;
+    if (ev <= 0.0f) { *scorethld = 999.0f; return CBP_OK; }
+    ev = logf(ev);
+    float sc = -ev / o->gap_lambda;
+    *scorethld = sc < 0.0f ? 0.0f : sc;
+    return CBP_OK;
+}
+
+}  /* extern "C" */
+
+/* device copy of a query's preparation (cached per query index) */
+DevQuery* c2_get_query(cbp_handle* h, int qidx, int* rc)
+{
+    auto it = h->qcache.find(qidx);
+    if (it != h->qcache.end()) return it->second.get();
+    auto dq = std::make_unique<DevQuery>();
+    c2_prepare_query(&h->queries.v, qidx, h->ssss.data(), h->rmax, &dq->hp);
+    const QueryPrep& p = dq->hp;
+    if ((*rc = dupload(h, dq->qv, p.qv.data(), p.qv.size()))) return nullptr;
+    if ((*rc = dupload(h, dq->qtr, p.qtr.data(), p.qtr.size()))) return nullptr;
+    if ((*rc = dupload(h, dq->QT, p.QT.data(), p.QT.size()))) return nullptr;
+    if ((*rc = dupload(h, dq->qh, p.qh.data(), p.qh.size()))) return nullptr;
+    if ((*rc = dupload(h, dq->lleft, p.lleft.data(), p.lleft.size()))) return nullptr;
+    if ((*rc = dupload(h, dq->ldiag, p.ldiag.data(), p.ldiag.size()))) return nullptr;
+    if ((*rc = dupload(h, dq->qbkg, p.qbkg.data(), p.qbkg.size()))) return nullptr;
+    DevQuery* r = dq.get();
+    h->qcache[qidx] = std::move(dq);
+    *rc = CBP_OK;
+    return r;
+}
+
+int c2_fill_p1args(cbp_handle* h, DevQuery* dq, P1Args* a)
+{
+    DevSet& d = h->dchunk;
+    std::memset(a, 0, sizeof *a);
+    a->trgfrq = d.trgfrq.as<float>(); a->ssprb = d.ssprb.as<float>();
+    a->trans = d.trans.as<float>(); a->bkg = d.bkg.as<float>();
+    a->len = d.len.as<int32_t>(); a->dist = d.dist.as<uint32_t>();
+    a->addrpro = d.addrpro.as<int32_t>();
+    a->npos = d.npos; a->nprof = d.nprof;
+    a->qv = dq->qv.as<float>(); a->qtr = dq->qtr.as<float>(); a->QT = dq->QT.as<float>();
+    a->qh = dq->qh.as<float>(); a->lleft = dq->lleft.as<float>(); a->ldiag = dq->ldiag.as<float>();
+    a->qbkg = dq->qbkg.as<float>();
+    a->Lq = dq->hp.Lq; a->Lpad = dq->hp.Lpad; a->nstripes = dq->hp.nstripes;
+    a->shift = h->opt.mapaln ? 0.002f : 0.0003f;
+    a->ssswgt = h->opt.ssswgt;
+    a->seg_p = h->dseg.as<int32_t>(); a->nseg = h->nseg;
+    a->keys = h->keys.as<unsigned long long>();
+    if (dq->hp.nstripes > 1) {
+        int rc = dalloc(h, h->bot, sizeof(float) * 5 * (size_t)d.npos);
+        if (rc) return rc;
+        a->bot = h->bot.as<float>();
+    }
+    return CBP_OK;
+}
+
+/* phase-1 launch for one query into h->keys; records kernel time */
+int c2_run_phase1(cbp_handle* h, int qidx)
+{
+    if (!h->have_opt || !h->have_tables || !h->have_queries || !h->have_chunk)
+        return fail(h, CBP_ERR_STATE, "options, tables, queries and chunk must be uploaded first");
+    if (qidx < 0 || qidx >= h->queries.v.nprof) return fail(h, CBP_ERR_ARG, "query index out of range");
+    (void)hipSetDevice(h->device);
+    int rc;
+    DevQuery* dq = c2_get_query(h, qidx, &rc);
+    if (!dq) return rc;
+    P1Args a;
+    if ((rc = c2_fill_p1args(h, dq, &a))) return rc;
+    int nw = h->nw;
+    while (nw > 1 && c2_phase1_lds_bytes(dq->hp.R, nw, dq->hp.nstripes) > 160 * 1024) nw >>= 1;
+    HIPCHK(hipMemsetAsync(h->keys.p, 0, sizeof(unsigned long long) * h->dchunk.nprof, h->stream));
+    HIPCHK(hipEventRecord(h->ev0, h->stream));
+    HIPCHK(c2_launch_phase1(a, dq->hp.R, nw, false, h->stream));
+    HIPCHK(hipEventRecord(h->ev1, h->stream));
+    HIPCHK(hipEventSynchronize(h->ev1));
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
+    h->last_ms = ms; h->last_nlaunch = 1;
+    return CBP_OK;
+}
+
+extern "C" {
+
+int cbp_phase1_keys(cbp_handle* h, int32_t qidx, uint64_t* keys)
+{
+    if (!h || !keys) return CBP_ERR_ARG;
+    int rc = c2_run_phase1(h, qidx);
+    if (rc) return rc;
+    HIPCHK(hipMemcpy(keys, h->keys.p, sizeof(uint64_t) * h->dchunk.nprof, hipMemcpyDeviceToHost));
+    return CBP_OK;
+}
+
+int cbp_last_kernel_ms(const cbp_handle* h, double* ms, int32_t* nlaunch)
+{
+    if (!h || !ms) return CBP_ERR_ARG;
+    *ms = h->last_ms;
+    if (nlaunch) *nlaunch = h->last_nlaunch;
+    return CBP_OK;
+}
+
+/* tuning knobs (autotuner): target segment columns, max rows per lane, waves per group */
+int cbp_set_tuning(cbp_handle* h, int32_t seg_cols, int32_t rmax, int32_t nw)
+{
+    if (!h || seg_cols < 64 || rmax < 1 || rmax > 5 || !(nw == 1 || nw == 2 || nw == 4 || nw == 8))
+        return CBP_ERR_ARG;
+    h->seg_cols = seg_cols; h->rmax = rmax; h->nw = nw;
+    h->qcache.clear();
+    return CBP_OK;
+}
+
+}  /* extern "C" */

@@ -1,0 +1,152 @@
+"""Python mirror of the reference's CuBatchProcessing interface over the C ABI.
+
+The method set follows cupro/CuBatchProcessing.cuh:50-112
+(TransferCPMData, SetDbDetails, SetQueryLen, SetScoreThreshold,
+SetLogEThreshold, ProcessScoreMatrix); every call goes to
+``libcomer2_hip.so`` through the functions declared in include/comer2_hip.h.
+There is no CPU fallback: constructing the object without the library or a
+GPU raises.
+"""
+import ctypes as C
+import os
+import numpy as np
+
+from . import _ffi
+from ._ffi import Profiles, Options, Results
+
+_lib = None
+
+
+class CbpError(RuntimeError):
+    pass
+
+
+def lib():
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(_ffi.LIB_HIP):
+        raise CbpError("libcomer2_hip.so is not built (run comer2_amd.build)")
+    L = C.CDLL(_ffi.LIB_HIP)
+    H = C.c_void_p
+    L.cbp_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(H)]
+    L.cbp_destroy.argtypes = [H]
+    L.cbp_last_error.argtypes = [H]
+    L.cbp_last_error.restype = C.c_char_p
+    L.cbp_set_options.argtypes = [H, C.POINTER(Options)]
+    L.cbp_upload_tables.argtypes = [H, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32]
+    L.cbp_upload_queries.argtypes = [H, C.POINTER(Profiles)]
+    L.cbp_upload_chunk.argtypes = [H, C.POINTER(Profiles)]
+    L.cbp_set_db_details.argtypes = [H, C.c_uint64, C.c_uint64]
+    L.cbp_score_threshold.argtypes = [C.POINTER(Options), C.c_uint64, C.c_uint64,
+                                      C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.cbp_phase1_keys.argtypes = [H, C.c_int32, np.ctypeslib.ndpointer(np.uint64, flags="C")]
+    L.cbp_last_kernel_ms.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
+    L.cbp_set_tuning.argtypes = [H, C.c_int32, C.c_int32, C.c_int32]
+    for name in ("cbp_process",):
+        if hasattr(L, name):
+            getattr(L, name).argtypes = [H, C.c_int32, C.c_float, C.c_float, C.POINTER(Results)]
+    _lib = L
+    return L
+
+
+def _fptr(a):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+class CuBatchProcessing:
+    """One device worker (the reference constructs one per DevCommThread)."""
+
+    def __init__(self, device=0, mem_budget=0, area=0, options=None, ssss=None, nnw=None):
+        from . import tables
+        self.L = lib()
+        self.h = C.c_void_p()
+        rc = self.L.cbp_create(device, mem_budget, area, C.byref(self.h))
+        if rc != 0:
+            raise CbpError("cbp_create failed (%d): no usable GPU or HIP error" % rc)
+        self.opts = dict(tables.default_options(), **(options or {}))
+        self._o = _ffi.options_struct(self.opts)
+        self._check(self.L.cbp_set_options(self.h, C.byref(self._o)))
+        ss = np.ascontiguousarray(tables.load_ssss() if ssss is None else ssss, np.float32)
+        nw = np.ascontiguousarray(tables.load_nn_weights() if nnw is None else nnw, np.float32)
+        self._check(self.L.cbp_upload_tables(self.h, _fptr(ss), _fptr(nw), nw.size))
+        self.scorethld = None
+        self.logevthld = None
+        self.dblen = 0
+        self.nseqs = 0
+        self.qlen = 0
+
+    def _check(self, rc):
+        if rc != 0:
+            raise CbpError("comer2 C ABI error %d: %s" % (rc, self.L.cbp_last_error(self.h).decode()))
+
+    def close(self):
+        if self.h:
+            self.L.cbp_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- reference method set ---
+    def TransferQueries(self, qset):
+        self._queries = qset
+        self._check(self.L.cbp_upload_queries(self.h, C.byref(qset.view)))
+
+    def TransferCPMData(self, dbset):
+        self._chunk = dbset
+        self._check(self.L.cbp_upload_chunk(self.h, C.byref(dbset.view)))
+
+    def SetDbDetails(self, dblen, nseqs):
+        self.dblen, self.nseqs = int(dblen), int(nseqs)
+        self._check(self.L.cbp_set_db_details(self.h, self.dblen, self.nseqs))
+
+    def SetQueryLen(self, qlen):
+        self.qlen = int(qlen)
+
+    def SetScoreThreshold(self, thld):
+        self.scorethld = float(thld)
+
+    def SetLogEThreshold(self, thld):
+        self.logevthld = float(thld)
+
+    def score_threshold(self, qlen, dblen):
+        s, l = C.c_float(), C.c_float()
+        self._check(self.L.cbp_score_threshold(C.byref(self._o), qlen, dblen, C.byref(s), C.byref(l)))
+        return s.value, l.value
+
+    # --- stage entry points ---
+    def phase1_keys(self, qidx):
+        keys = np.zeros(self._chunk.nprof, np.uint64)
+        self._check(self.L.cbp_phase1_keys(self.h, qidx, keys))
+        return keys
+
+    def last_kernel_ms(self):
+        ms, n = C.c_double(), C.c_int32()
+        self._check(self.L.cbp_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))
+        return ms.value, n.value
+
+    def set_tuning(self, seg_cols=4096, rmax=4, nw=8):
+        self._check(self.L.cbp_set_tuning(self.h, seg_cols, rmax, nw))
+
+
+def decode_keys(keys):
+    """(score float32, x, y) arrays from phase-1 keys."""
+    keys = np.asarray(keys, np.uint64)
+    sc = (keys >> np.uint64(32)).astype(np.uint32).view(np.float32)
+    lo = np.uint32(0xFFFFFFFF) - (keys & np.uint64(0xFFFFFFFF)).astype(np.uint32)
+    x = (lo >> np.uint32(16)).astype(np.int32)
+    y = (lo & np.uint32(0xFFFF)).astype(np.int32)
+    z = keys == 0
+    x[z] = 0
+    y[z] = 0
+    return sc, x, y
+
+
+def encode_key(score, x, y):
+    hi = int(np.float32(score).view(np.uint32))
+    lo = 0xFFFFFFFF - ((int(x) << 16) | int(y))
+    return (hi << 32) | lo if score > 0 else 0

@@ -1,0 +1,150 @@
+/*
+ * comer2_hip.h -- C ABI of the MI355X (gfx950) libmycu search path.
+ *
+ * The entry points replace the device side of the reference's
+ * CuBatchProcessing (cupro/CuBatchProcessing.cuh:47-244) as driven by
+ * DevCommThread (cualn/DevCommThread.cpp:149-356).  Plain C: opaque handle,
+ * borrowed host pointers (copied before return), int status codes
+ * (0 = success), no exceptions across the boundary, one handle per worker
+ * thread (calls on one handle are not thread-safe; distinct handles are).
+ */
+#ifndef COMER2_HIP_H
+#define COMER2_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define CBP_OK            0
+#define CBP_ERR_ARG      -1   /* invalid argument */
+#define CBP_ERR_HIP      -2   /* HIP runtime failure */
+#define CBP_ERR_MEM      -3   /* device memory budget exceeded */
+#define CBP_ERR_STATE    -4   /* call out of order (e.g. no query uploaded) */
+#define CBP_ERR_NODEV    -5   /* no usable GPU */
+
+/* number of alignment-data fields per passed pair; field order of
+ * TDP2OutputAlnData, cuss/CuBatchSS_com.h:45-71 */
+enum {
+    CBP_F_SCORE = 0, CBP_F_ORGPRONO, CBP_F_PRONEWDST, CBP_F_PSTVS,
+    CBP_F_BEGCOORDS, CBP_F_ENDCOORDS,
+    CBP_F_LMBDEST, CBP_F_KEST, CBP_F_EVALUE, CBP_F_EPA,
+    CBP_F_MIN, CBP_F_MAX, CBP_F_E, CBP_F_H, CBP_F_LMBD, CBP_F_K,
+    CBP_F_IDNTS, CBP_F_NGAPS,
+    CBP_NFIELDS
+};
+
+/* SoA profile set; mirrors the fields of PM2DVectorFields
+ * (cupro/PM2DVectorFields.h:48-81).  Field-major arrays:
+ *   bkg    [20][nprof]      background probabilities (pps2DBkgPrbs)
+ *   eno    [nprof]          effective number of observations (pps2DENO)
+ *   len    [nprof]          profile length (pps2DLen)
+ *   dist   [nprof]          first position of each profile (pps2DDist)
+ *   trgfrq [20][npos]       target frequencies (pmv2DTrgFrqs)
+ *   ssprb  [3][npos]        SS state probabilities (pmv2DSSsprbs)
+ *   aa     [npos]           residue letters (pmv2Daa)
+ *   sss    [npos]           SS state (pmv2DSSstate)
+ *   trans  [7][npos+nprof]  log transitions MM MI MD IM II DM DD, len+1 rows
+ *                           per profile, row 0 = beginning (ptr2DTrnPrbs)  */
+typedef struct {
+    int32_t         nprof;
+    int64_t         npos;
+    const float*    bkg;
+    const float*    eno;
+    const int32_t*  len;
+    const uint32_t* dist;
+    const float*    trgfrq;
+    const float*    ssprb;
+    const char*     aa;
+    const char*     sss;
+    const float*    trans;
+} cbp_profiles;
+
+/* options (src/env/options.txt) and statistical parameters (src/env/comer.par) */
+typedef struct {
+    float   ssswgt;     /* SSSWGT  options.txt:81 */
+    int32_t mapaln;     /* MAPALN  options.txt:126 */
+    float   minpp;      /* MINPP   options.txt:131 */
+    float   evalue;     /* EVAL    options.txt:9 */
+    int32_t ssemodel;   /* SSEMODEL options.txt:93 */
+    int32_t xuninf;     /* X_UNINF options.txt:28 */
+    float   ung_lambda, ung_K, ung_H;             /* [UNGAPPED]  comer.par:5-11 */
+    float   gap_lambda, gap_K, gap_alpha, gap_beta; /* [GAPPED_A_1] comer.par:15-20 */
+    int32_t max_map_pairs; /* cap on realigned pairs per query (<=0: no cap) */
+} cbp_options;
+
+/* per-query search outcome; buffers are owned by the handle and stay valid
+ * until the next cbp_process* call on it */
+typedef struct {
+    int32_t      npassed;    /* pairs passing phase 1 (DP + correlation) */
+    int64_t      nposits;    /* sum of passed profile lengths */
+    int32_t      maxlen;     /* longest passed profile */
+    int32_t      nmap;       /* pairs realigned by MAP (first nmap of order) */
+    const float* alndata;    /* [npassed][CBP_NFIELDS] in profile order */
+    const int32_t* order;    /* [npassed] indices into alndata sorted by (log E, profile index) */
+    const char*  alns;       /* alignment strings: per realigned pair, 3 rows
+                                of alnstride chars (query, middle, target) */
+    int64_t      alnstride;
+    float        scorethld, logevthld;
+    double       search_space;
+} cbp_results;
+
+typedef struct cbp_handle cbp_handle;
+
+/* lifecycle (replaces CuDeviceMemory construction, CuDeviceMemory.cu:87-140) */
+int  cbp_create(int device, size_t mem_budget, int area, cbp_handle** out);
+int  cbp_destroy(cbp_handle* h);
+const char* cbp_last_error(const cbp_handle* h);
+
+/* options + tables (CuDeviceMemory::CacheCompleteData, CuDeviceMemory.cu:583-640):
+ * ssss900: the SS score table expanded to 30x30 (src/env/ssss);
+ * nnw: SS18 network weights + adjustment parameters (93 floats,
+ * CuDeviceMemory.cu:40-80 layout CuBatchProcessing_com.cuh:44-94) */
+int  cbp_set_options(cbp_handle* h, const cbp_options* opt);
+int  cbp_upload_tables(cbp_handle* h, const float* ssss900, const float* nnw, int32_t nnw_count);
+
+/* data (CuDeviceMemory::TransferCPMDataDevice, CuDeviceMemory.cu:1036-1142;
+ * CuBatchProcessing::TransferCPMData) */
+int  cbp_upload_queries(cbp_handle* h, const cbp_profiles* q);
+int  cbp_upload_chunk(cbp_handle* h, const cbp_profiles* db);
+/* whole-database size used for thresholds and E-values
+ * (CuBatchProcessing::SetDbDetails) */
+int  cbp_set_db_details(cbp_handle* h, uint64_t dblen, uint64_t nseqs);
+
+/* score threshold and log E threshold for a query of length qlen searched
+ * against dblen positions (JobDispatcher::CalculateScoreThreshold,
+ * cualn/JobDispatcher.cpp:809-860) */
+int  cbp_score_threshold(const cbp_options* opt, uint64_t qlen, uint64_t dblen,
+                         float* scorethld, float* logevthld);
+
+/* full per-(chunk, query) pipeline: score -> Viterbi -> statistics -> MAP
+ * (CuBatchProcessing::BatchProcessScoreMatrixDevice, CuBatchProcessing.cu:229-572) */
+int  cbp_process(cbp_handle* h, int32_t qidx, float scorethld, float logevthld,
+                 cbp_results* out);
+
+/* all listed queries against the uploaded chunk in one device run */
+int  cbp_process_batch(cbp_handle* h, const int32_t* qidx, int32_t nq,
+                       cbp_results* outs /* [nq] */);
+
+/* ---- stage entry points (exposed for parity tests and profiling) ---- */
+
+/* phase-1 fused score+Viterbi over all chunk profiles for query qidx;
+ * keys[p] = (float bits of max MM score) << 32 | ~(x << 16 | y), x, y the
+ * profile-local coordinates of the maximum (0 if no positive cell) */
+int  cbp_phase1_keys(cbp_handle* h, int32_t qidx, uint64_t* keys /* [nprof] */);
+
+/* FP32 score matrix S and its SS component for query qidx vs chunk profile
+ * pronr, row-major [qlen][dblen] (CalcSM_Init + CalcSM_SSSS) */
+int  cbp_score_block(cbp_handle* h, int32_t qidx, int32_t pronr, float* S, float* mod);
+
+/* device-side elapsed time of the last cbp_phase1_keys / cbp_process call's
+ * dominant kernel (ms, HIP events on its stream) and the number of launches */
+int  cbp_last_kernel_ms(const cbp_handle* h, double* ms, int32_t* nlaunch);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* COMER2_HIP_H */

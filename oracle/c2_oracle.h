@@ -1,0 +1,84 @@
+/*
+ * c2_oracle.h -- CPU restatement of the reference's libmycu search path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  Nothing in the product (comer2_amd/, bench.py's
+ * timed legs) links or calls this code; it is the checker used by tests/,
+ * __graft_entry__.smoke() and bench.py's cpu_baseline leg.
+ *
+ * Two arithmetic modes:
+ *   - FP32 "mirror": the exact operation order the gfx950 kernels use (the
+ *     reference's recurrences, our fixed FP32 score formulation); backtrack
+ *     decisions, coordinates and paths must match the kernels bit for bit.
+ *   - FP64: the reference's formulas in double precision; FP32 values are
+ *     compared with it within a stated tolerance.
+ * Each function cites the reference file:line it restates.
+ */
+#ifndef C2_ORACLE_H
+#define C2_ORACLE_H
+#include <stdint.h>
+#include "../include/comer2_hip.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* tables */
+typedef struct {
+    float ssss[30][30];     /* SS score table (src/env/ssss), symmetric */
+    float nnw[93];          /* SS18 weights, CuDeviceMemory.cu:40-80 */
+} or_tables;
+
+/* --- scores (cusco) --- */
+void or_score_f32(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
+                  const or_tables* t, const cbp_options* o, float* S, float* mod);
+void or_score_f64(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
+                  const or_tables* t, const cbp_options* o, double* S, double* mod);
+
+/* --- phase-1 Viterbi (cudp) --- */
+typedef struct {
+    float    maxsc;      /* max MM over the matrix (0 if none positive) */
+    int32_t  x, y;       /* coordinates of the maximum, canonical tie rule */
+} or_dpmax;
+
+/* column -1 boundary vectors for a query (5 states MM MI IM DN ND each):
+ * lleft[r] = state of (r,-1) as seen by (r,0); ldiag[r] = state of (r-1,-1)
+ * as seen by (r,0) */
+void or_colm1(const cbp_profiles* q, int qp, float (*lleft)[5], float (*ldiag)[5]);
+
+/* direct FP32 DP over a pair with the reference's semantics;
+ * S: [Lq][Ldb] with corners already masked (or NULL to compute);
+ * btck (optional) [Lq][Ldb]; mm (optional) [Lq][Ldb] */
+or_dpmax or_viterbi_f32(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
+                        const float* S, char* btck, float* mm);
+double   or_viterbi_f64(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
+                        const double* S);
+/* corner-masked copy of S (CuBatchDP_init_btck.cuh:117-170) */
+void or_mask_corners_f32(int Lq, int Ldb, float* S);
+void or_mask_corners_f64(int Lq, int Ldb, double* S);
+
+/* literal emulation of the reference's block-diagonal schedule
+ * (ExecDP_Btck_Unroll32x + FinalizeDPPro max reduction) for one pair;
+ * Sraw unmasked; returns the reference's own (maxsc, coords) choice */
+or_dpmax or_refemu_viterbi(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
+                           const float* Sraw, char* btck, float* mm);
+
+/* phase-1 finalize: backtrace + correlation (CuBatchDP_final.cu:61-292) */
+typedef struct {
+    float   score;       /* max score + signed sqrt correlation bonus */
+    int32_t psts;        /* positive substitution scores on DIAG steps */
+    int32_t begx, begy;  /* beginning (x+1, y+1) */
+    int32_t endx, endy;  /* end = max coordinates */
+} or_fin;
+or_fin or_finalize(int Lq, int Ldb, const float* S, const char* btck, or_dpmax m);
+
+/* per-cell backtrack codes */
+enum { OR_STOP = 0, OR_LEFT = 1, OR_UP = 2, OR_DIAG = 3 };
+
+/* score threshold (JobDispatcher.cpp:809-860) */
+void or_score_threshold(const cbp_options* o, uint64_t qlen, uint64_t dblen,
+                        float* scorethld, float* logevthld);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

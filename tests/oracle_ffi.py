@@ -1,0 +1,111 @@
+"""ctypes bindings to the oracle (oracle/liboracle.so) -- test infrastructure."""
+import ctypes as C
+import os
+import numpy as np
+from comer2_amd._ffi import Profiles, Options
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+LIB = os.path.join(ROOT, "oracle", "liboracle.so")
+
+
+class Tables(C.Structure):
+    _fields_ = [("ssss", (C.c_float * 30) * 30), ("nnw", C.c_float * 93)]
+
+
+class DpMax(C.Structure):
+    _fields_ = [("maxsc", C.c_float), ("x", C.c_int32), ("y", C.c_int32)]
+
+
+class Fin(C.Structure):
+    _fields_ = [("score", C.c_float), ("psts", C.c_int32), ("begx", C.c_int32),
+                ("begy", C.c_int32), ("endx", C.c_int32), ("endy", C.c_int32)]
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(LIB)
+        P = C.POINTER(Profiles)
+        fp = np.ctypeslib.ndpointer(np.float32, flags="C")
+        dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+        cp = np.ctypeslib.ndpointer(np.int8, flags="C")
+        T = C.POINTER(Tables)
+        O = C.POINTER(Options)
+        L.or_score_f32.argtypes = [P, C.c_int, P, C.c_int, T, O, fp, fp]
+        L.or_score_f64.argtypes = [P, C.c_int, P, C.c_int, T, O, dp, dp]
+        L.or_colm1.argtypes = [P, C.c_int, fp, fp]
+        L.or_viterbi_f32.argtypes = [P, C.c_int, P, C.c_int, fp, cp, fp]
+        L.or_viterbi_f32.restype = DpMax
+        L.or_viterbi_f64.argtypes = [P, C.c_int, P, C.c_int, dp]
+        L.or_viterbi_f64.restype = C.c_double
+        L.or_refemu_viterbi.argtypes = [P, C.c_int, P, C.c_int, fp, cp, fp]
+        L.or_refemu_viterbi.restype = DpMax
+        L.or_finalize.argtypes = [C.c_int, C.c_int, fp, cp, DpMax]
+        L.or_finalize.restype = Fin
+        L.or_mask_corners_f32.argtypes = [C.c_int, C.c_int, fp]
+        L.or_score_threshold.argtypes = [O, C.c_uint64, C.c_uint64,
+                                         C.POINTER(C.c_float), C.POINTER(C.c_float)]
+        _lib = L
+    return _lib
+
+
+def make_tables(ssss, nnw):
+    t = Tables()
+    arr = np.ctypeslib.as_array(t.ssss)
+    arr[:] = ssss
+    np.ctypeslib.as_array(t.nnw)[:] = nnw
+    return t
+
+
+class Oracle:
+    def __init__(self, ssss, nnw, opts):
+        self.L = lib()
+        self.t = make_tables(ssss, nnw)
+        self.o = Options()
+        for k, _ in Options._fields_:
+            setattr(self.o, k, opts[k])
+
+    def score_f32(self, q, qp, d, dp):
+        Lq, Ld = q.len[qp], d.len[dp]
+        S = np.empty((Lq, Ld), np.float32); M = np.empty((Lq, Ld), np.float32)
+        self.L.or_score_f32(C.byref(q), qp, C.byref(d), dp, C.byref(self.t), C.byref(self.o), S, M)
+        return S, M
+
+    def score_f64(self, q, qp, d, dp):
+        Lq, Ld = q.len[qp], d.len[dp]
+        S = np.empty((Lq, Ld), np.float64); M = np.empty((Lq, Ld), np.float64)
+        self.L.or_score_f64(C.byref(q), qp, C.byref(d), dp, C.byref(self.t), C.byref(self.o), S, M)
+        return S, M
+
+    def viterbi_f32(self, q, qp, d, dp, S, want_btck=True, want_mm=False):
+        Lq, Ld = q.len[qp], d.len[dp]
+        bt = np.zeros((Lq, Ld), np.int8); mm = np.zeros((Lq, Ld), np.float32)
+        r = self.L.or_viterbi_f32(C.byref(q), qp, C.byref(d), dp, np.ascontiguousarray(S), bt, mm)
+        return r, bt, mm
+
+    def refemu(self, q, qp, d, dp, S):
+        Lq, Ld = q.len[qp], d.len[dp]
+        bt = np.zeros((Lq, Ld), np.int8); mm = np.zeros((Lq, Ld), np.float32)
+        r = self.L.or_refemu_viterbi(C.byref(q), qp, C.byref(d), dp, np.ascontiguousarray(S), bt, mm)
+        return r, bt, mm
+
+    def viterbi_f64(self, q, qp, d, dp, S):
+        return self.L.or_viterbi_f64(C.byref(q), qp, C.byref(d), dp, np.ascontiguousarray(S))
+
+    def finalize(self, S, bt, m):
+        Lq, Ld = S.shape
+        return self.L.or_finalize(Lq, Ld, np.ascontiguousarray(S), bt, m)
+
+    def colm1(self, q, qp):
+        Lq = q.len[qp]
+        a = np.zeros((Lq, 5), np.float32); b = np.zeros((Lq, 5), np.float32)
+        self.L.or_colm1(C.byref(q), qp, a, b)
+        return a, b
+
+    def score_threshold(self, qlen, dblen):
+        s = C.c_float(); l = C.c_float()
+        self.L.or_score_threshold(C.byref(self.o), qlen, dblen, C.byref(s), C.byref(l))
+        return s.value, l.value
