@@ -24,7 +24,8 @@ struct P1Args {
     int32_t Lq, Lpad, nstripes;
     float shift, ssswgt;
     /* segments of whole profiles */
-    const int32_t* seg_p; /* [nseg+1] */
+    const int32_t* seg_pb; /* [nseg] first profile of each segment */
+    const int32_t* seg_pe; /* [nseg] one past the last profile */
     int32_t nseg;
     /* outputs */
     unsigned long long* keys;   /* [nprof] */
@@ -33,7 +34,41 @@ struct P1Args {
     const int64_t* btck_off;    /* per segment word offset */
 };
 
+/* everything needed to recompute one score-matrix cell */
+struct ScoreCtx {
+    const float* trgfrq; const float* ssprb; const float* bkg; const int32_t* len;
+    const uint32_t* dist; const char* aa;
+    int64_t npos; int32_t nprof;
+    const float* qv; const float* QT; const float* qh; const float* qbkg; const char* qaa;
+    int32_t Lq, Lpad, R, nstripes;
+    float shift, ssswgt;
+};
+
+/* finalize record of a phase-1 candidate */
+struct FinRec {
+    float score;        /* max + signed sqrt correlation */
+    float maxsc;        /* phase-1 maximum */
+    int32_t psts;
+    uint32_t beg, end;  /* CombineCoords(x, y) */
+    int32_t pad;
+};
+
 size_t c2_phase1_lds_bytes(int R, int NW, int nstripes);
+hipError_t c2_launch_finalize(const ScoreCtx& c, const int32_t* cand, int32_t ncand,
+                              const unsigned long long* keys, const uint32_t* btck,
+                              const int64_t* btck_off, FinRec* out, hipStream_t stream);
+hipError_t c2_launch_hist(const ScoreCtx& c, const int32_t* pro, int32_t npro, int xuninf,
+                          int32_t* counts /* [npro][641] */, hipStream_t stream);
+struct StatIn {
+    int32_t Ldb; float deno; float score; float psts;
+};
+struct StatParams {
+    int32_t Lq; float qeno; float searchspace; int32_t ssemodel;
+    float ung_lambda, ung_K, gap_lambda, gap_K;
+};
+hipError_t c2_launch_stats(const StatParams& sp, const float* nnw /*device [93]*/,
+                           const int32_t* counts, const StatIn* in, int32_t n,
+                           double* out /* [n][10] */, hipStream_t stream);
 hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream_t stream);
 
 #endif

@@ -131,6 +131,9 @@ int cbp_upload_tables(cbp_handle* h, const float* ssss900, const float* nnw, int
     if (nnw) {
         if (nnw_count != 93) return fail(h, CBP_ERR_ARG, "expected 93 SS18 parameters");
         h->nnw.assign(nnw, nnw + 93);
+        int rc = dupload(h, h->dnnw, h->nnw.data(), 93);
+        if (rc) return rc;
+        HIPCHK(hipStreamSynchronize(h->stream));
     }
     h->have_tables = true;
     h->qcache.clear();
@@ -181,7 +184,8 @@ int cbp_upload_chunk(cbp_handle* h, const cbp_profiles* db)
     }
     if (h->seg_p.back() != s.nprof) h->seg_p.push_back(s.nprof);
     h->nseg = (int32_t)h->seg_p.size() - 1;
-    if ((rc = dupload(h, h->dseg, h->seg_p.data(), h->seg_p.size()))) return rc;
+    if ((rc = dupload(h, h->dseg_b, h->seg_p.data(), h->nseg))) return rc;
+    if ((rc = dupload(h, h->dseg_e, h->seg_p.data() + 1, h->nseg))) return rc;
     if ((rc = dalloc(h, h->keys, sizeof(unsigned long long) * np))) return rc;
     h->chunk.copy_from(db);
     HIPCHK(hipStreamSynchronize(h->stream));
@@ -231,6 +235,12 @@ DevQuery* c2_get_query(cbp_handle* h, int qidx, int* rc)
     if ((*rc = dupload(h, dq->lleft, p.lleft.data(), p.lleft.size()))) return nullptr;
     if ((*rc = dupload(h, dq->ldiag, p.ldiag.data(), p.ldiag.size()))) return nullptr;
     if ((*rc = dupload(h, dq->qbkg, p.qbkg.data(), p.qbkg.size()))) return nullptr;
+    {
+        std::vector<char> qa((size_t)p.Lpad, 'X');
+        const size_t q0 = h->queries.v.dist[qidx];
+        std::memcpy(qa.data(), h->queries.v.aa + q0, (size_t)p.Lq);
+        if ((*rc = dupload(h, dq->qaa, qa.data(), qa.size()))) return nullptr;
+    }
     DevQuery* r = dq.get();
     h->qcache[qidx] = std::move(dq);
     *rc = CBP_OK;
@@ -252,7 +262,7 @@ int c2_fill_p1args(cbp_handle* h, DevQuery* dq, P1Args* a)
     a->Lq = dq->hp.Lq; a->Lpad = dq->hp.Lpad; a->nstripes = dq->hp.nstripes;
     a->shift = h->opt.mapaln ? 0.002f : 0.0003f;
     a->ssswgt = h->opt.ssswgt;
-    a->seg_p = h->dseg.as<int32_t>(); a->nseg = h->nseg;
+    a->seg_pb = h->dseg_b.as<int32_t>(); a->seg_pe = h->dseg_e.as<int32_t>(); a->nseg = h->nseg;
     a->keys = h->keys.as<unsigned long long>();
     if (dq->hp.nstripes > 1) {
         int rc = dalloc(h, h->bot, sizeof(float) * 5 * (size_t)d.npos);
@@ -317,3 +327,196 @@ int cbp_set_tuning(cbp_handle* h, int32_t seg_cols, int32_t rmax, int32_t nw)
 }
 
 }  /* extern "C" */
+
+/* ----------------------------------------------------------------------- */
+/* full pipeline                                                            */
+
+/* AbstractScoreMatrix::ComputeLengthAdjustment (libpro/srcsco/AbstractScoreMatrix.cpp:
+ * 247-390): Altschul-Gertz iteration in float, as the reference computes it */
+uint64_t c2_length_adjustment(const cbp_options& o, uint64_t m, uint64_t n, uint64_t N, uint64_t* delta)
+{
+    const float lambda = o.gap_lambda, K = o.gap_K, alpha = o.gap_alpha, beta = o.gap_beta;
+    uint64_t ss = m * n;
+    *delta = 0;
+    if (lambda <= 0.0f || K <= 0.0f || N < 1) return ss;
+    const float logK = logf(K), alratio = alpha / lambda;
+    const float max_mn = (float)((m > n) ? m : n);
+    float length = 0.0f, min_length = 0.0f, max_length = 0.0f, nxt_length = 0.0f;
+    float sspace = 0.0f, space2 = 0.0f;
+    bool converged = false;
+    const float a = (float)N, b = (float)(n + m * N), c = (float)(m * n) - max_mn / K;
+    if (c < 0.0f) return ss;
+    max_length = 2.0f * c / (b + sqrtf(b * b - 4.0f * a * c));
+    for (int j = 0; j < 20; j++) {
+        length = nxt_length;
+        sspace = ((float)m - length) * ((float)n - (float)N * length);
+        nxt_length = alratio * (logK + logf(sspace)) + beta;
+        if (length <= nxt_length) {
+            min_length = length;
+            if (nxt_length - length <= 1.0f) { converged = true; break; }
+            if (min_length == max_length) break;
+        } else max_length = length;
+        if (nxt_length < min_length || max_length < nxt_length)
+            nxt_length = (!j) ? max_length : (min_length + max_length) * 0.5f;
+    }
+    if (converged) {
+        length = ceilf(min_length);
+        if (length <= max_length) {
+            space2 = ((float)m - length) * ((float)n - (float)N * length);
+            nxt_length = alratio * (logK + logf(space2)) + beta;
+            if (length <= nxt_length) { min_length = length; sspace = space2; }
+        }
+    }
+    *delta = (uint64_t)min_length;
+    return (uint64_t)sspace;
+}
+
+static void fill_scorectx(cbp_handle* h, DevQuery* dq, ScoreCtx* c)
+{
+    DevSet& d = h->dchunk;
+    c->trgfrq = d.trgfrq.as<float>(); c->ssprb = d.ssprb.as<float>(); c->bkg = d.bkg.as<float>();
+    c->len = d.len.as<int32_t>(); c->dist = d.dist.as<uint32_t>(); c->aa = d.aa.as<char>();
+    c->npos = d.npos; c->nprof = d.nprof;
+    c->qv = dq->qv.as<float>(); c->QT = dq->QT.as<float>(); c->qh = dq->qh.as<float>();
+    c->qbkg = dq->qbkg.as<float>(); c->qaa = dq->qaa.as<char>();
+    c->Lq = dq->hp.Lq; c->Lpad = dq->hp.Lpad; c->R = dq->hp.R; c->nstripes = dq->hp.nstripes;
+    c->shift = h->opt.mapaln ? 0.002f : 0.0003f;
+    c->ssswgt = h->opt.ssswgt;
+}
+
+static uint32_t f2u(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
+static float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
+
+int c2_process_one(cbp_handle* h, int32_t qidx, float scorethld, float logevthld, cbp_results* out)
+{
+    int rc = c2_run_phase1(h, qidx);
+    if (rc) return rc;
+    DevQuery* dq = c2_get_query(h, qidx, &rc);
+    const int np = h->dchunk.nprof;
+    const cbp_profiles& ch = h->chunk.v;
+    /* 1. candidates: phase-1 maximum above the score threshold */
+    std::vector<unsigned long long> keys(np);
+    HIPCHK(hipMemcpy(keys.data(), h->keys.p, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost));
+    std::vector<int32_t> cand;
+    for (int p = 0; p < np; p++)
+        if (keys[p] && u2f((uint32_t)(keys[p] >> 32)) > scorethld) cand.push_back(p);
+    const int nc = (int)cand.size();
+    std::vector<FinRec> fin(nc);
+    const int R = dq->hp.R, ns = dq->hp.nstripes;
+    if (nc > 0) {
+        std::vector<int64_t> off(nc + 1, 0);
+        std::vector<int32_t> pe(nc);
+        for (int i = 0; i < nc; i++) {
+            off[i + 1] = off[i] + (int64_t)ns * (ch.len[cand[i]] + 63) * 64;
+            pe[i] = cand[i] + 1;
+        }
+        if ((rc = dupload(h, h->cand, cand.data(), nc))) return rc;
+        if ((rc = dupload(h, h->cand_off, off.data(), nc + 1))) return rc;
+        if ((rc = dalloc(h, h->btck, sizeof(uint32_t) * (size_t)off[nc]))) return rc;
+        DBuf dpe;
+        if ((rc = dupload(h, dpe, pe.data(), nc))) return rc;
+        /* 2. recompute the candidates' DP with backtrack codes */
+        P1Args a;
+        if ((rc = c2_fill_p1args(h, dq, &a))) return rc;
+        a.seg_pb = h->cand.as<int32_t>(); a.seg_pe = dpe.as<int32_t>(); a.nseg = nc;
+        a.btck = h->btck.as<uint32_t>(); a.btck_off = h->cand_off.as<int64_t>();
+        HIPCHK(c2_launch_phase1(a, R, 1, true, h->stream));
+        /* 3. backtrace + correlation */
+        if ((rc = dalloc(h, h->fin, sizeof(FinRec) * nc))) return rc;
+        ScoreCtx sc;
+        fill_scorectx(h, dq, &sc);
+        HIPCHK(c2_launch_finalize(sc, h->cand.as<int32_t>(), nc, h->keys.as<unsigned long long>(),
+                                  h->btck.as<uint32_t>(), h->cand_off.as<int64_t>(),
+                                  h->fin.as<FinRec>(), h->stream));
+        HIPCHK(hipMemcpyAsync(fin.data(), h->fin.p, sizeof(FinRec) * nc, hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        HIPCHK(hipFree(dpe.p)); dpe.p = nullptr; h->used -= dpe.n;
+    }
+    /* 4. passed list in profile order (canonical replacement of the spin-locked append) */
+    std::vector<int32_t> pass;
+    std::vector<int> passfin;
+    for (int i = 0; i < nc; i++)
+        if (scorethld < fin[i].score) { pass.push_back(cand[i]); passfin.push_back(i); }
+    const int npass = (int)pass.size();
+    h->res_aln.assign((size_t)npass * CBP_NFIELDS, 0.0f);
+    int64_t nposits = 0; int32_t maxlen = 0;
+    for (int k = 0; k < npass; k++) {
+        const FinRec& f = fin[passfin[k]];
+        float* r = &h->res_aln[(size_t)k * CBP_NFIELDS];
+        r[CBP_F_SCORE] = f.score;
+        r[CBP_F_ORGPRONO] = u2f((uint32_t)pass[k]);
+        r[CBP_F_PRONEWDST] = u2f((uint32_t)nposits);
+        r[CBP_F_PSTVS] = (float)f.psts;
+        r[CBP_F_BEGCOORDS] = u2f(f.beg);
+        r[CBP_F_ENDCOORDS] = u2f(f.end);
+        nposits += ch.len[pass[k]];
+        if (ch.len[pass[k]] > maxlen) maxlen = ch.len[pass[k]];
+    }
+    /* 5. statistics */
+    const uint64_t dblen = h->dblen ? h->dblen : (uint64_t)ch.npos;
+    const uint64_t nseqs = h->nseqs ? h->nseqs : (uint64_t)ch.nprof;
+    uint64_t delta = 0;
+    const uint64_t sspace = c2_length_adjustment(h->opt, (uint64_t)dq->hp.Lq, dblen, nseqs, &delta);
+    if (npass > 0) {
+        if ((rc = dupload(h, h->pass_pro, pass.data(), npass))) return rc;
+        if ((rc = dalloc(h, h->counts, sizeof(int32_t) * 641 * (size_t)npass))) return rc;
+        ScoreCtx sc;
+        fill_scorectx(h, dq, &sc);
+        HIPCHK(c2_launch_hist(sc, h->pass_pro.as<int32_t>(), npass, h->opt.xuninf, h->counts.as<int32_t>(), h->stream));
+        std::vector<StatIn> si(npass);
+        for (int k = 0; k < npass; k++) {
+            si[k].Ldb = ch.len[pass[k]];
+            si[k].deno = ch.eno[pass[k]];
+            si[k].score = h->res_aln[(size_t)k * CBP_NFIELDS + CBP_F_SCORE];
+            si[k].psts = h->res_aln[(size_t)k * CBP_NFIELDS + CBP_F_PSTVS];
+        }
+        if ((rc = dupload(h, h->statin, si.data(), npass))) return rc;
+        if ((rc = dalloc(h, h->statout, sizeof(double) * 10 * (size_t)npass))) return rc;
+        StatParams sp;
+        sp.Lq = dq->hp.Lq;
+        sp.qeno = h->queries.v.eno[qidx];
+        sp.searchspace = (float)sspace;
+        sp.ssemodel = h->opt.ssemodel;
+        sp.ung_lambda = h->opt.ung_lambda; sp.ung_K = h->opt.ung_K;
+        sp.gap_lambda = h->opt.gap_lambda; sp.gap_K = h->opt.gap_K;
+        if (!h->dnnw.p) return fail(h, CBP_ERR_STATE, "SS18 parameters not uploaded");
+        HIPCHK(c2_launch_stats(sp, h->dnnw.as<float>(), h->counts.as<int32_t>(), h->statin.as<StatIn>(),
+                               npass, h->statout.as<double>(), h->stream));
+        std::vector<double> so((size_t)10 * npass);
+        HIPCHK(hipMemcpyAsync(so.data(), h->statout.p, sizeof(double) * so.size(), hipMemcpyDeviceToHost, h->stream));
+        HIPCHK(hipStreamSynchronize(h->stream));
+        for (int k = 0; k < npass; k++) {
+            float* r = &h->res_aln[(size_t)k * CBP_NFIELDS];
+            const double* o = &so[(size_t)k * 10];
+            r[CBP_F_MIN] = (float)o[0]; r[CBP_F_MAX] = (float)o[1]; r[CBP_F_E] = (float)o[2];
+            r[CBP_F_H] = (float)o[3]; r[CBP_F_LMBD] = (float)o[4]; r[CBP_F_K] = (float)o[5];
+            r[CBP_F_LMBDEST] = (float)o[6]; r[CBP_F_KEST] = (float)o[7];
+            r[CBP_F_EVALUE] = (float)o[8]; r[CBP_F_EPA] = (float)o[9];
+        }
+    }
+    /* 6. order by (log E, profile index) */
+    h->res_order.resize(npass);
+    for (int k = 0; k < npass; k++) h->res_order[k] = k;
+    std::stable_sort(h->res_order.begin(), h->res_order.end(), [&](int a, int b) {
+        return h->res_aln[(size_t)a * CBP_NFIELDS + CBP_F_EVALUE] < h->res_aln[(size_t)b * CBP_NFIELDS + CBP_F_EVALUE];
+    });
+    std::memset(out, 0, sizeof *out);
+    out->npassed = npass;
+    out->nposits = nposits;
+    out->maxlen = maxlen;
+    out->nmap = 0;
+    out->alndata = h->res_aln.data();
+    out->order = h->res_order.data();
+    out->alns = nullptr;
+    out->alnstride = 0;
+    out->scorethld = scorethld;
+    out->logevthld = logevthld;
+    out->search_space = (double)sspace;
+    return CBP_OK;
+}
+
+extern "C" int cbp_process(cbp_handle* h, int32_t qidx, float scorethld, float logevthld, cbp_results* out)
+{
+    if (!h || !out) return CBP_ERR_ARG;
+    return c2_process_one(h, qidx, scorethld, logevthld, out);
+}

@@ -37,7 +37,7 @@ struct DevSet {
 
 struct DevQuery {
     QueryPrep hp;
-    DBuf qv, qtr, QT, qh, lleft, ldiag, qbkg;
+    DBuf qv, qtr, QT, qh, lleft, ldiag, qbkg, qaa;
 };
 
 struct cbp_handle {
@@ -58,7 +58,10 @@ struct cbp_handle {
     DevSet dchunk;
     bool have_chunk = false;
     std::vector<int32_t> seg_p;
-    DBuf dseg;
+    DBuf dseg_b, dseg_e;
+    DBuf dnnw;           /* SS18 parameters on the device */
+    /* per-call scratch */
+    DBuf cand, cand_off, btck, fin, pass_pro, counts, statin, statout;
     int32_t nseg = 0;
     DBuf keys;           /* [nprof] u64 */
     DBuf bot;            /* stripe hand-off */

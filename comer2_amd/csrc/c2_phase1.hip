@@ -114,7 +114,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
         __syncthreads();
         if (seg >= a.nseg) continue;
 
-        const int pb = a.seg_p[seg], pe = a.seg_p[seg + 1];
+        const int pb = a.seg_pb[seg], pe = a.seg_pe[seg];
         const int64_t c_begin = a.dist[pb];
         const int64_t c_end = (int64_t)a.dist[pe - 1] + a.len[pe - 1];
         const int ncols = (int)(c_end - c_begin);

@@ -1,0 +1,530 @@
+/*
+ * c2_phase2.hip -- phase-1 finalize (backtrace + correlation bonus) and the
+ * alignment statistics for gfx950.
+ *
+ * k_finalize replaces FinalizeDP/FinalizeDPPro (cudp/CuBatchDP_final.cu:61-396):
+ * one thread per candidate pair walks the packed backtrack codes written by
+ * the BTCK phase-1 launch from the maximum cell, collects the DIAG scores and
+ * accumulates the correlation sum in the reference's order (32-score windows
+ * reduced by an xor butterfly, windows added in sequence), then adds
+ * sign*sqrt(|0.5 sum|).  The passed list is compacted on the host in profile
+ * order (the reference appends under a spin lock in block-scheduling order).
+ *
+ * k_hist replaces InitScoreProbsDynPlmProfile + CalcScoreProbsXuninfDynPlmProfile
+ * (cuss/CuBatchSS_dynplm.cu:61-182): one workgroup per passed pair recomputes
+ * the pair's score matrix and bins rint(32*clamp(S,-10,10)) into an LDS
+ * integer histogram (exact counts instead of float atomics).
+ *
+ * k_stats replaces CalcStatParamsDynPlmProfile and the SS18/Obs significance
+ * kernels (cuss/CuBatchSS_dynplm.cu:193-674, cuss/CuBatchSS_{lambda,entropy,K,
+ * SS18,NN}.cuh, cusf/*): one thread per pair, FP64 throughout.
+ */
+#include "c2_dev.h"
+#include "c2_kernels.h"
+
+namespace {
+
+/* score-matrix cell (y, col) of query vs profile p; the operation order is
+ * the one of the phase-1 kernel (c2_phase1.hip) */
+__device__ __forceinline__ float score_at(const ScoreCtx& c, int y, int64_t col, int p)
+{
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 20; k++) {
+        const float w = 1.0f / (0.5f * (c.qbkg[k] + c.bkg[(size_t)k * c.nprof + p]));
+        const float dw = c.trgfrq[(size_t)k * c.npos + col] * w;
+        const float qk = c.qv[(size_t)k * c.Lpad + y];
+        acc = (k == 0) ? qk * dw : fmaf(qk, dw, acc);
+    }
+    float sc = c2_logf_clamped(acc) - c.shift;
+    if (c.ssswgt > 0.0f) {
+        float pj[3]; int co[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            pj[j] = c.ssprb[(size_t)j * c.npos + col];
+            co[j] = 10 * j + (int)(pj[j] * 9.999f);
+        }
+        const float dh = (pj[2] + 0.1f) * 0.1111111f;
+        float ss = c.QT[(size_t)co[0] * c.Lpad + y] * pj[0];
+        ss = fmaf(c.QT[(size_t)co[1] * c.Lpad + y], pj[1], ss);
+        ss = fmaf(c.QT[(size_t)co[2] * c.Lpad + y], pj[2], ss);
+        ss = fmaf(c.qh[y], dh, ss);
+        ss = (ss - 0.05f) * c.ssswgt;
+        sc = sc + ss;
+    }
+    return sc;
+}
+
+__global__ void k_finalize(ScoreCtx c, const int32_t* cand, int32_t ncand,
+                           const unsigned long long* keys, const uint32_t* btck,
+                           const int64_t* btck_off, FinRec* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= ncand) return;
+    const int p = cand[i];
+    const unsigned long long key = keys[p];
+    const float maxsc = __uint_as_float((uint32_t)(key >> 32));
+    const uint32_t lo = 0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFull);
+    const int xe = (int)(lo >> 16), ye = (int)(lo & 0xFFFF);
+    const int Ldb = c.len[p], nsteps = Ldb + 63, SL = 64 * c.R;
+    const int64_t col0 = c.dist[p];
+    const uint32_t* bt = btck + btck_off[i];
+    int x = xe, y = ye, psts = 0;
+    int b = 3;
+    float cache[36];
+    for (int k = 0; k < 36; k++) cache[k] = 0.0f;
+    float sumcorr = 0.0f;
+    if (key != 0ull) {
+        while (b != 0) {
+            for (int k = 0; k < 4; k++) cache[k] = cache[k + 32];
+            int ndx;
+            for (ndx = 4; ndx < 36;) {
+                if (x < 0 || y < 0) { b = 0; break; }
+                const int s = y / SL, rr = y - s * SL, l = rr / c.R, kk = rr - l * c.R;
+                const uint32_t w = bt[((size_t)s * nsteps + (x + l)) * 64 + l];
+                b = (int)((w >> (2 * kk)) & 3u);
+                if (b == 0) break;
+                if (b == 2) { y--; continue; }
+                if (b == 1) { x--; continue; }
+                const float sv = score_at(c, y, col0 + x, p);
+                cache[ndx] = sv;
+                if (sv > 0.0f) psts++;
+                x--; y--; ndx++;
+            }
+            float v[32];
+            for (int t = 0; t < 32; t++)
+                v[t] = (ndx <= t + 4) ? 0.0f
+                     : cache[t + 4] * (cache[t] + cache[t + 1] + cache[t + 2] + cache[t + 3]);
+            for (int off = 16; off >= 1; off >>= 1) {
+                float n[32];
+                for (int t = 0; t < 32; t++) n[t] = v[t] + v[t ^ off];
+                for (int t = 0; t < 32; t++) v[t] = n[t];
+            }
+            sumcorr += v[0];
+        }
+    }
+    sumcorr *= 0.5f;
+    sumcorr = sumcorr < 0.0f ? -__fsqrt_rn(-sumcorr) : __fsqrt_rn(sumcorr);
+    FinRec r;
+    r.score = maxsc + sumcorr;
+    r.maxsc = maxsc;
+    r.psts = psts;
+    r.beg = ((uint32_t)(x + 1) << 16) | (uint32_t)(y + 1);
+    r.end = ((uint32_t)xe << 16) | (uint32_t)ye;
+    r.pad = 0;
+    out[i] = r;
+}
+
+constexpr int NSC = 641;
+
+__global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, int xuninf, int32_t* counts)
+{
+    __shared__ int h[NSC];
+    for (int i = threadIdx.x; i < NSC; i += blockDim.x) h[i] = 0;
+    __syncthreads();
+    const int p = pro[blockIdx.x];
+    const int Ldb = c.len[p];
+    const int64_t col0 = c.dist[p];
+    for (int x = threadIdx.x; x < Ldb; x += blockDim.x) {
+        const int64_t col = col0 + x;
+        if (xuninf && c.aa[col] == 'X') continue;
+        float dw[20];
+#pragma unroll
+        for (int k = 0; k < 20; k++) {
+            const float w = 1.0f / (0.5f * (c.qbkg[k] + c.bkg[(size_t)k * c.nprof + p]));
+            dw[k] = c.trgfrq[(size_t)k * c.npos + col] * w;
+        }
+        float pj[3]; int co[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            pj[j] = c.ssprb[(size_t)j * c.npos + col];
+            co[j] = 10 * j + (int)(pj[j] * 9.999f);
+        }
+        const float dh = (pj[2] + 0.1f) * 0.1111111f;
+        for (int y = 0; y < c.Lq; y++) {
+            if (xuninf && c.qaa[y] == 'X') continue;
+            float acc = c.qv[y] * dw[0];
+#pragma unroll
+            for (int k = 1; k < 20; k++) acc = fmaf(c.qv[(size_t)k * c.Lpad + y], dw[k], acc);
+            float sc = c2_logf_clamped(acc) - c.shift;
+            if (c.ssswgt > 0.0f) {
+                float ss = c.QT[(size_t)co[0] * c.Lpad + y] * pj[0];
+                ss = fmaf(c.QT[(size_t)co[1] * c.Lpad + y], pj[1], ss);
+                ss = fmaf(c.QT[(size_t)co[2] * c.Lpad + y], pj[2], ss);
+                ss = fmaf(c.qh[y], dh, ss);
+                ss = (ss - 0.05f) * c.ssswgt;
+                sc = sc + ss;
+            }
+            if (sc < -10.0f) sc = -10.0f; else if (sc > 10.0f) sc = 10.0f;
+            const int bin = (int)rintf(sc * 32.0f) + 320;
+            atomicAdd(&h[bin], 1);
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < NSC; i += blockDim.x)
+        counts[(size_t)blockIdx.x * NSC + i] = h[i];
+}
+
+/* ---------------- FP64 statistics (one thread per pair) ---------------- */
+
+constexpr int SMIN = -10, NUNS = 21, SCL = 32, SSMIN = -320;
+constexpr double LOG_SP_MIN = -8.7336545e+01, LOG_SP_MAX = 8.8722839e+01;
+constexpr double LN1K = 6.90775527898213681510242167860;
+constexpr double MACHEP = 1.11022302462515654042e-16;
+constexpr double BIG = 4.503599627370496e15, BIGINV = 2.22044604925031308085e-16;
+
+__device__ void interval(int y, int* l, int* r) {
+    const int u = y + SMIN;
+    int left = u * SCL - (SCL >> 1) - SSMIN, rght = u * SCL + (SCL >> 1) - SSMIN;
+    if (u <= 0) { left++; if (left < 0) left = 0; }
+    if (0 <= u) { rght--; if (NSC - 1 < rght) rght = NSC - 1; }
+    *l = left; *r = rght;
+}
+
+__device__ void lambda_f(double x, const double* P, const int* L, double mn, double mx, double* f, double* df) {
+    double lf = 0.0, ldf = 0.0;
+    for (int y = 0; y < NUNS; y++)
+        for (int t = 0; t < SCL; t++) {
+            double sc = L[y] + t + SSMIN;
+            if (mn <= sc && sc <= mx) {
+                sc -= mx;
+                const double v = x * sc;
+                if (LOG_SP_MIN < v) { const double e = exp(v) * P[y * SCL + t]; lf += e; ldf += e * sc; }
+            }
+        }
+    double z = x * -mx;
+    if (LOG_SP_MIN < z) { z = exp(z); lf -= z; ldf += z * mx; }
+    *f = lf; if (df) *df = ldf;
+}
+
+__device__ double find_lambda(const double* P, const int* L, double mn, double mx) {
+    const double x1 = 1.0e-5, x2 = 10.0, acc = 1.0e-4;
+    double f, df, fl, dx, dxold, xh, xl, rts;
+    lambda_f(x1, P, L, mn, mx, &fl, nullptr);
+    lambda_f(x2, P, L, mn, mx, &f, nullptr);
+    if ((fl > 0.0 && f > 0.0) || (fl < 0.0 && f < 0.0)) return -1.0;
+    if (fl < 0.0) { xl = x1; xh = x2; } else { xh = x1; xl = x2; }
+    rts = 0.5 * (x1 + x2); dxold = fabs(x2 - x1); dx = dxold;
+    lambda_f(rts, P, L, mn, mx, &f, &df);
+    int n;
+    for (n = 0; n < 32; n++) {
+        bool stop = false;
+        if ((((rts - xh) * df - f) * ((rts - xl) * df - f) > 0.0) || (fabs(2.0 * f) > fabs(dxold * df))) {
+            dxold = dx; dx = 0.5 * (xh - xl); rts = xl + dx;
+            if (xl == rts) stop = true;
+        } else {
+            dxold = dx; dx = f / df;
+            const double tmp = rts; rts -= dx;
+            if (tmp == rts) stop = true;
+        }
+        if (fabs(dx) < acc) stop = true;
+        if (stop) break;
+        lambda_f(rts, P, L, mn, mx, &f, &df);
+        if (f < 0.0) xl = rts; else xh = rts;
+    }
+    return n == 32 ? -1.0 : rts;
+}
+
+__device__ double prob_len_j(double* C, double* nw, const double* pu, const int* L, int minu, int maxu,
+                             int range, int minsc, int maxsc, double lambda) {
+    double pj = 0.0;
+    for (int y = 0; y < NUNS; y++)
+        for (int t = 0; t < SCL; t++) {
+            double pji = 0.0;
+            const int sc = L[y] + t + SSMIN;
+            if (minsc <= sc && sc <= maxsc && -32 <= sc && sc <= 32) {
+                const double etols = (sc < 0) ? lambda * (double)sc : 0.0;
+                if (LOG_SP_MIN < etols) {
+                    const int lo = (maxsc - sc < range) ? maxu - (maxsc - sc) : minu;
+                    const int hi = (sc - minsc < range) ? minu + (sc - minsc) : maxu;
+                    for (int i = lo; i <= hi; i++) {
+                        int yy = y, xx = t - i;
+                        if (SCL <= xx) { xx -= SCL; if (yy + 1 < NUNS) yy++; else continue; }
+                        else if (xx < 0) { xx += SCL; if (0 < yy) yy--; else continue; }
+                        pji += C[yy * SCL + xx] * pu[i - SMIN];
+                    }
+                    pj += pji * exp(etols);
+                }
+            }
+            nw[y * SCL + t] = pji;
+        }
+    for (int i = 0; i < NUNS * SCL; i++) C[i] = nw[i];
+    return pj;
+}
+
+__device__ double incbcf(double a, double b, double x) {
+    double xk, pk, pkm1, pkm2, qk, qkm1, qkm2, k1, k2, k3, k4, k5, k6, k7, k8, r, t, ans;
+    k1 = a; k2 = a + b; k3 = a; k4 = a + 1.0; k5 = 1.0; k6 = b - 1.0; k7 = k4; k8 = a + 2.0;
+    pkm2 = 0.0; qkm2 = 1.0; pkm1 = 1.0; qkm1 = 1.0; ans = 1.0; r = 1.0;
+    for (int n = 0; n < 100; n++) {
+        xk = -(x * k1 * k2) / (k3 * k4);
+        pk = pkm1 + pkm2 * xk; qk = qkm1 + qkm2 * xk; pkm2 = pkm1; pkm1 = pk; qkm2 = qkm1; qkm1 = qk;
+        xk = (x * k5 * k6) / (k7 * k8);
+        pk = pkm1 + pkm2 * xk; qk = qkm1 + qkm2 * xk; pkm2 = pkm1; pkm1 = pk; qkm2 = qkm1; qkm1 = qk;
+        if (qk != 0.0) r = pk / qk;
+        if (r != 0.0) { t = fabs((ans - r) / r); ans = r; } else t = 1.0;
+        if (t < 3.0 * MACHEP) break;
+        k1 += 1.0; k2 += 1.0; k3 += 2.0; k4 += 2.0; k5 += 1.0; k6 -= 1.0; k7 += 2.0; k8 += 2.0;
+        if (fabs(qk) + fabs(pk) > BIG) { pkm2 *= BIGINV; pkm1 *= BIGINV; qkm2 *= BIGINV; qkm1 *= BIGINV; }
+        if (fabs(qk) < BIGINV || fabs(pk) < BIGINV) { pkm2 *= BIG; pkm1 *= BIG; qkm2 *= BIG; qkm1 *= BIG; }
+    }
+    return ans;
+}
+__device__ double incbd(double a, double b, double x) {
+    double xk, pk, pkm1, pkm2, qk, qkm1, qkm2, k1, k2, k3, k4, k5, k6, k7, k8, r, t, ans, z;
+    k1 = a; k2 = b - 1.0; k3 = a; k4 = a + 1.0; k5 = 1.0; k6 = a + b; k7 = a + 1.0; k8 = a + 2.0;
+    pkm2 = 0.0; qkm2 = 1.0; pkm1 = 1.0; qkm1 = 1.0; z = x / (1.0 - x); ans = 1.0; r = 1.0;
+    for (int n = 0; n < 100; n++) {
+        xk = -(z * k1 * k2) / (k3 * k4);
+        pk = pkm1 + pkm2 * xk; qk = qkm1 + qkm2 * xk; pkm2 = pkm1; pkm1 = pk; qkm2 = qkm1; qkm1 = qk;
+        xk = (z * k5 * k6) / (k7 * k8);
+        pk = pkm1 + pkm2 * xk; qk = qkm1 + qkm2 * xk; pkm2 = pkm1; pkm1 = pk; qkm2 = qkm1; qkm1 = qk;
+        if (qk != 0.0) r = pk / qk;
+        if (r != 0.0) { t = fabs((ans - r) / r); ans = r; } else t = 1.0;
+        if (t < 3.0 * MACHEP) break;
+        k1 += 1.0; k2 -= 1.0; k3 += 2.0; k4 += 2.0; k5 += 1.0; k6 += 1.0; k7 += 2.0; k8 += 2.0;
+        if (fabs(qk) + fabs(pk) > BIG) { pkm2 *= BIGINV; pkm1 *= BIGINV; qkm2 *= BIGINV; qkm1 *= BIGINV; }
+        if (fabs(qk) < BIGINV || fabs(pk) < BIGINV) { pkm2 *= BIG; pkm1 *= BIG; qkm2 *= BIG; qkm1 *= BIG; }
+    }
+    return ans;
+}
+__device__ double lpseries(double a, double b, double x) {
+    double y = a * log(x) + (b - 1.0) * log(1.0 - x) - log(a);
+    y -= lgamma(a) + lgamma(b);
+    y += lgamma(a + b);
+    double t = x / (1.0 - x), s = 0.0, u = 1.0;
+    do { b -= 1.0; if (b == 0.0) break; a += 1.0; u *= t * b / a; s += u; } while (fabs(u) > MACHEP);
+    return y + log(1.0 + s);
+}
+__device__ double lbeta_helper(double a, double b, double x) {
+    double ans, t = 1.0 - x;
+    if (b > 10.0 && fabs(b * x / a) < 0.3) return lpseries(a, b, x);
+    ans = x * (a + b - 2.0) / (a - 1.0);
+    if (ans < 1.0) { ans = incbcf(a, b, x); t = b * log(t); }
+    else { ans = incbd(a, b, x); t = (b - 1.0) * log(t); }
+    t += a * log(x) + lgamma(a + b) - lgamma(a) - lgamma(b);
+    t += log(ans / a);
+    return t;
+}
+__device__ double lbetainc(double aa, double bb, double xx) {
+    if (xx <= 0.0 || xx >= 1.0) { if (xx == 0.0) return -999.0; if (xx == 1.0) return 0.0; return 999.0; }
+    if (aa <= 0.0) return 999.0;
+    if (aa <= 1.0) {
+        double ans = lbeta_helper(aa + 1.0, bb, xx);
+        ans = ans > LOG_SP_MIN ? exp(ans) : 0.0;
+        const double t = aa * log(xx) + bb * log(1.0 - xx) + lgamma(aa + bb) - lgamma(aa + 1.0) - lgamma(bb);
+        if (t > LOG_SP_MIN) ans += exp(t);
+        return ans != 0.0 ? log(ans) : -999.0;
+    }
+    return lbeta_helper(aa, bb, xx);
+}
+__device__ double lgammainc_P(double a, double x);
+__device__ double lgammainc_Q(double a, double x) {
+    if (x <= 0.0 || a <= 0.0) return 0.0;
+    if (x < 1.0 || x < a) {
+        double ans = lgammainc_P(a, x);
+        ans = ans > LOG_SP_MIN ? exp(ans) : 0.0;
+        return ans < 1.0 ? log(1.0 - ans) : -999.0;
+    }
+    const double ax = a * log(x) - x - lgamma(a);
+    double y = 1.0 - a, z = x + y + 1.0, c = 0.0, pkm2 = 1.0, qkm2 = x, pkm1 = x + 1.0, qkm1 = z * x;
+    double ans = pkm1 / qkm1, t, r, pk, qk, yc;
+    do {
+        c += 1.0; y += 1.0; z += 2.0; yc = y * c;
+        pk = pkm1 * z - pkm2 * yc; qk = qkm1 * z - qkm2 * yc;
+        if (qk != 0.0) { r = pk / qk; t = fabs((ans - r) / r); ans = r; } else t = 1.0;
+        pkm2 = pkm1; pkm1 = pk; qkm2 = qkm1; qkm1 = qk;
+        if (fabs(pk) > BIG) { pkm2 *= BIGINV; pkm1 *= BIGINV; qkm2 *= BIGINV; qkm1 *= BIGINV; }
+    } while (t > MACHEP);
+    return log(ans) + ax;
+}
+__device__ double lgammainc_P(double a, double x) {
+    if (x <= 0.0 || a <= 0.0) return -999.0;
+    if (x > 1.0 && x > a) {
+        double ans = lgammainc_Q(a, x);
+        ans = ans > LOG_SP_MIN ? exp(ans) : 0.0;
+        return ans < 1.0 ? log(1.0 - ans) : -999.0;
+    }
+    const double ax = a * log(x) - x - lgamma(a);
+    double r = a, c = 1.0, ans = 1.0;
+    do { r += 1.0; c *= x / r; ans += c; } while (c / ans > MACHEP);
+    return log(ans) + ax - log(a);
+}
+
+__device__ double clampd(double v, double lo, double hi) { return v < lo ? lo : (v > hi ? hi : v); }
+
+__device__ void nnet(const double in[4], const float* w, double factor, double* out) {
+    double a[4];
+    for (int h = 0; h < 4; h++) {
+        const double z = (double)w[h * 5] + in[0] * w[h * 5 + 1] + in[1] * w[h * 5 + 2] +
+                         in[2] * w[h * 5 + 3] + in[3] * w[h * 5 + 4];
+        a[h] = (2.0 / (1.0 + exp(-z)) - 1.0) * w[20 + h + 1];
+    }
+    const double o = (double)w[20] + a[0] + a[1] + a[2] + a[3];
+    *out = (2.0 / (1.0 + exp(-o)) - 1.0) * factor;
+}
+
+__global__ __launch_bounds__(64) void k_stats(StatParams sp, const float* nnw, const int32_t* counts,
+                                              const StatIn* in, int32_t n, double* out)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    double P[NUNS * SCL], NW[NUNS * SCL], pu[NUNS];
+    int L[NUNS], Rr[NUNS];
+    const int32_t* cnt = counts + (size_t)i * NSC;
+    double tot = 0.0;
+    for (int y = 0; y < NUNS; y++) {
+        interval(y, &L[y], &Rr[y]);
+        double s = 0.0;
+        for (int t = 0; t < SCL; t++) {
+            P[y * SCL + t] = (t + L[y] <= Rr[y]) ? (double)cnt[L[y] + t] : 0.0;
+            s += P[y * SCL + t];
+        }
+        pu[y] = s; tot += s;
+    }
+    for (int y = 0; y < NUNS; y++) { pu[y] /= tot; for (int t = 0; t < SCL; t++) P[y * SCL + t] /= tot; }
+    double E = 0.0, mn = 0.0, mx = 0.0;
+    for (int y = 0; y < NUNS; y++)
+        for (int t = 0; t < SCL; t++) {
+            const double sc = L[y] + t + SSMIN;
+            E += sc * P[y * SCL + t];
+            const double v = P[y * SCL + t] != 0.0 ? sc : 0.0;
+            if (v < mn) mn = v;
+            if (v > mx) mx = v;
+        }
+    double lmb = (0.0 <= E || mx <= 0.0) ? -1.0 : find_lambda(P, L, mn, mx);
+    double H = -1.0;
+    if (lmb > 0.0) {
+        H = 0.0;
+        for (int y = 0; y < NUNS; y++)
+            for (int t = 0; t < SCL; t++) {
+                const double sc = L[y] + t + SSMIN;
+                if (mn <= sc && sc <= mx) {
+                    const double v = lmb * sc;
+                    if (LOG_SP_MIN < v && v < LOG_SP_MAX - 3.0) H += exp(v) * v * P[y * SCL + t];
+                }
+            }
+    }
+    for (int t = 0; t < SCL; t++) P[(0 - SMIN) * SCL + t] = 1.0;
+    if (lmb > 0.0) lmb *= SCL;
+    E /= SCL;
+    int minu = 0, maxu = 0;
+    for (int k = 0; k < NUNS; k++) {
+        const int v = pu[k] != 0.0 ? k + SMIN : 0;
+        if (v < minu) minu = v;
+        if (v > maxu) maxu = v;
+    }
+    double K;
+    if (0.0 <= E || lmb <= 0.0 || H <= 0.0) K = -1.0;
+    else if (minu == -1 && maxu == 1) { const double a = pu[-1 - SMIN], b = pu[1 - SMIN]; K = (a - b) * (a - b) / a; }
+    else if (minu == -1 || maxu == 1) {
+        if (maxu != 1) K = lmb / H * (1.0 - exp(-lmb)) * E * E;
+        else K = H / lmb * (1.0 - exp(-lmb));
+    } else {
+        double sigma = 0.0, Pj = 1.0;
+        int minsc = minu, maxsc = maxu;
+        const int range = maxu - minu + 1;
+        for (int j = 1; j <= 32 && 1.0e-4 < Pj; j++) {
+            Pj = prob_len_j(P, NW, pu, L, minu, maxu, range, minsc, maxsc, lmb);
+            sigma += Pj / j;
+            minsc += minu; maxsc += maxu;
+        }
+        K = lmb * exp(-2.0 * sigma) / (H * (1.0 - exp(-lmb)));
+    }
+    double* o = out + (size_t)i * 10;
+    o[0] = minu; o[1] = maxu; o[2] = E; o[3] = H; o[4] = lmb; o[5] = K;
+    const double lref = sp.ung_lambda, kref = sp.ung_K, lg = sp.gap_lambda, kg = sp.gap_K;
+    const double sc = in[i].score, ss = sp.searchspace;
+    const int Ldb = in[i].Ldb;
+    o[6] = lmb > 0.0 ? lmb * lg / lref : 0.0;
+    o[7] = K > 0.0 ? K * kg / kref : 0.0;
+    const double l1l2 = (double)(float)((uint32_t)sp.Lq * (uint32_t)Ldb);
+    o[9] = (lmb <= 0.0 || K <= 0.0) ? log(kref * l1l2) - lref * sc : log(K * l1l2) - lmb * sc;
+    if (sp.ssemodel == 0) { o[8] = log(kg * ss) - lg * sc; return; }
+    double inp[4], eMU, lMU, lSC;
+    {
+        double qe = sp.qeno, de = in[i].deno;
+        uint32_t lq = sp.Lq, ld = Ldb;
+        if (qe < de) { const double t = qe; qe = de; de = t; const uint32_t u = lq; lq = ld; ld = u; }
+        else if (qe == de && lq < ld) { const uint32_t u = lq; lq = ld; ld = u; }
+        inp[0] = 0.05 * qe; inp[1] = log((double)lq) / LN1K; inp[2] = 0.05 * de; inp[3] = log((double)ld) / LN1K;
+        nnet(inp, nnw, 30.0, &eMU);
+    }
+    {
+        double lm = rint(lmb * 10.0) / 10.0;
+        if (lm <= 0.0) lm = 0.1;
+        if (10.0 < lm) lm = 10.0;
+        uint32_t lq = sp.Lq, ld = Ldb;
+        if (lq < ld) { const uint32_t u = lq; lq = ld; ld = u; }
+        inp[0] = 0.2 * lm; inp[1] = log((double)lq) / LN1K; inp[2] = log((double)ld) / LN1K; inp[3] = 0.0;
+        nnet(inp, nnw + 25, 30.0, &lMU);
+        nnet(inp, nnw + 50, 5.0, &lSC);
+    }
+    const float* adj = nnw + (sp.ssemodel == 2 ? 84 : 75);
+    if (eMU < 3.0) eMU = 3.0;
+    if (lMU < 3.0) lMU = 3.0;
+    if (lSC < 0.1) lSC = 0.1;
+    lSC = clampd(adj[6] * (exp(lSC) - 1.0), 0.1, 5.0);
+    const double scaleE = clampd(adj[4] * eMU + adj[5], 0.1, 5.0);
+    eMU = adj[0] * eMU + adj[1];
+    lMU = adj[2] * lMU + adj[3];
+    double alpha = adj[7], ibeta = adj[8];
+    if (0.0 < lmb) {
+        const double rc = 1.0 / lmb, m10 = rc < 10.0 ? rc : 10.0;
+        alpha -= alpha * 0.1 * m10;
+        ibeta += (1.0 - ibeta) * 0.1 * m10;
+    }
+    const double mu = alpha * eMU + (1.0 - alpha) * lMU;
+    const double scale = (1.0 - ibeta) * scaleE + ibeta * lSC;
+    if (mu <= 0.0 || scale <= 0.0) { o[8] = log(kg * ss) - lg * sc; return; }
+    double logKmn = mu / scale;
+    if (K > kref && 0.0 < K && K < 1.0 && 0.0 < kref && kref < 1.0) logKmn += log(K / kref);
+    double logexp = logKmn - sc / scale;
+    if (sp.ssemodel == 2) logexp += -9.210340372;
+    double logp = logexp;
+    if (-4.605170186 <= logexp) {
+        if (LOG_SP_MAX - 3.0 < logexp) logp = 0.0;
+        else {
+            const double pp = exp(logexp);
+            logp = (LOG_SP_MAX - 3.0 < pp) ? 0.0 : log(1.0 - exp(-pp));
+        }
+    }
+    const double l12 = l1l2 < 625.0 ? 625.0 : l1l2;
+    const double pn = rint(100000.0 * (double)in[i].psts / pow(l12, 1.5));
+    const double lpp = lbetainc(pn + 1.0, 0.1, 0.757213);
+    logp = lgammainc_Q(1.346206, (-logp - lpp) / 1.485656);
+    logexp = logp;
+    if (-4.605170186 <= logp) {
+        const double pp = exp(logp);
+        logexp = 77.0;
+        if (pp < 1.0) logexp = log(-log(1.0 - pp));
+    }
+    o[8] = logexp + log(ss / l1l2);
+}
+
+}  // namespace
+
+hipError_t c2_launch_finalize(const ScoreCtx& c, const int32_t* cand, int32_t ncand,
+                              const unsigned long long* keys, const uint32_t* btck,
+                              const int64_t* btck_off, FinRec* out, hipStream_t stream)
+{
+    if (ncand <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_finalize, dim3((ncand + 63) / 64), dim3(64), 0, stream, c, cand, ncand, keys, btck, btck_off, out);
+    return hipGetLastError();
+}
+
+hipError_t c2_launch_hist(const ScoreCtx& c, const int32_t* pro, int32_t npro, int xuninf,
+                          int32_t* counts, hipStream_t stream)
+{
+    if (npro <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_hist, dim3(npro), dim3(256), 0, stream, c, pro, xuninf, counts);
+    return hipGetLastError();
+}
+
+hipError_t c2_launch_stats(const StatParams& sp, const float* nnw, const int32_t* counts,
+                           const StatIn* in, int32_t n, double* out, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_stats, dim3((n + 63) / 64), dim3(64), 0, stream, sp, nnw, counts, in, n, out);
+    return hipGetLastError();
+}

@@ -124,6 +124,18 @@ class CuBatchProcessing:
         self._check(self.L.cbp_phase1_keys(self.h, qidx, keys))
         return keys
 
+    def ProcessScoreMatrix(self, qidx, scorethld=None, logevthld=None):
+        """Full search of query qidx against the uploaded chunk; returns a dict of
+        numpy arrays (one row per pair that passed phase 1, profile order)."""
+        if scorethld is None or logevthld is None:
+            qlen = self._queries.lengths()[qidx]
+            s, l = self.score_threshold(qlen, self.dblen or self._chunk.npos)
+            scorethld = s if scorethld is None else scorethld
+            logevthld = l if logevthld is None else logevthld
+        r = Results()
+        self._check(self.L.cbp_process(self.h, qidx, scorethld, logevthld, C.byref(r)))
+        return results_to_dict(r)
+
     def last_kernel_ms(self):
         ms, n = C.c_double(), C.c_int32()
         self._check(self.L.cbp_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))
@@ -131,6 +143,32 @@ class CuBatchProcessing:
 
     def set_tuning(self, seg_cols=4096, rmax=4, nw=8):
         self._check(self.L.cbp_set_tuning(self.h, seg_cols, rmax, nw))
+
+
+def results_to_dict(r):
+    n = r.npassed
+    if n > 0:
+        a = np.ctypeslib.as_array(r.alndata, (n, _ffi.NFIELDS)).copy()
+        order = np.ctypeslib.as_array(r.order, (n,)).copy()
+    else:
+        a = np.zeros((0, _ffi.NFIELDS), np.float32)
+        order = np.zeros(0, np.int32)
+    u = a.view(np.uint32)
+    out = dict(npassed=n, nposits=r.nposits, maxlen=r.maxlen, nmap=r.nmap,
+               scorethld=r.scorethld, logevthld=r.logevthld, search_space=r.search_space,
+               alndata=a, order=order,
+               score=a[:, _ffi.F_SCORE], pronr=u[:, _ffi.F_ORGPRONO].astype(np.int64),
+               dst=u[:, _ffi.F_PRONEWDST].astype(np.int64), psts=a[:, _ffi.F_PSTVS],
+               beg=u[:, _ffi.F_BEGCOORDS], end=u[:, _ffi.F_ENDCOORDS],
+               logE=a[:, _ffi.F_EVALUE], logEpA=a[:, _ffi.F_EPA],
+               lmbd=a[:, _ffi.F_LMBD], K=a[:, _ffi.F_K], H=a[:, _ffi.F_H], E=a[:, _ffi.F_E],
+               smin=a[:, _ffi.F_MIN], smax=a[:, _ffi.F_MAX],
+               lmbdest=a[:, _ffi.F_LMBDEST], kest=a[:, _ffi.F_KEST],
+               idnts=a[:, _ffi.F_IDNTS], ngaps=a[:, _ffi.F_NGAPS])
+    if r.nmap > 0 and r.alns:
+        out["alns"] = np.ctypeslib.as_array(C.cast(r.alns, C.POINTER(C.c_uint8)),
+                                            (r.nmap, 3, r.alnstride)).copy()
+    return out
 
 
 def decode_keys(keys):

@@ -82,3 +82,35 @@ void or_score_threshold(const cbp_options* o, uint64_t qlen, uint64_t dblen,
 }
 #endif
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* --- statistics (cuss, cusf) in FP64 --- */
+typedef struct {
+    double Min, Max, E, H, Lmbd, K;          /* TDP2OutputAlnData Min..K (unscaled) */
+    double LmbdEst, KEst, Evalue, EpA;       /* phase-2 significance fields (logs) */
+} or_stats;
+
+/* histogram of rint(32*clamp(S,-10,10)) (CalcScoreProbsXuninfDynPlmProfile,
+ * cuss/CuBatchSS_dynplm.cu:80-182); counts[641] */
+void or_histogram(const float* S, int Lq, int Ldb, const char* qaa, const char* daa,
+                  int xuninf, int64_t* counts);
+/* CalcStatParamsDynPlmProfile + CalcSignificanceSS18/Obs (cuss/CuBatchSS_dynplm.cu:193-674) */
+or_stats or_statistics(const int64_t* counts, int Lq, int Ldb, float qeno, float deno,
+                       float score, float psts, float searchspace,
+                       const cbp_options* o, const float* nnw);
+/* AbstractScoreMatrix::ComputeLengthAdjustment (libpro/srcsco/AbstractScoreMatrix.cpp:247-390),
+ * float arithmetic as in the reference; returns the (uint64) search space */
+uint64_t or_length_adjustment(const cbp_options* o, uint64_t m, uint64_t n, uint64_t N,
+                              uint64_t* delta);
+/* special functions (cusf/CuSF_betainc.cuh, cusf/CuSF_gammainc.cuh), FP64 */
+double or_lbetainc(double a, double b, double x);
+double or_lgammainc_Q(double a, double x);
+double or_lgammainc_P(double a, double x);
+/* pinning hooks: 1 = disable the reference's column-(-1) zero quirk rows
+ * (planted mistake used only to show that tests can see it) */
+void or_set_flags(int flags);
+#ifdef __cplusplus
+}
+#endif

@@ -207,8 +207,12 @@ static st5 cell_f32(const st5* dg, const st5* up, const st5* lf,
  * in the next block and reads zeros (x-1<0 branch, CuBatchDP_init_btck.cu:
  * 203-224); its diagonal neighbour (r-1,-1) is seen as zeros on rows
  * r%32==0 (bottom buffer, :228-240) and r%32==31.  This restates that. */
+static int g_flags = 0;
+void or_set_flags(int flags) { g_flags = flags; }
+
 void or_colm1(const cbp_profiles* q, int qp, float (*lleft)[5], float (*ldiag)[5])
 {
+    const int noquirk = g_flags & 1;
     const int Lq = q->len[qp];
     st5 zero = {{0, 0, 0, 0, 0}};
     float dt[7]; trn7_log0(dt);
@@ -224,8 +228,8 @@ void or_colm1(const cbp_profiles* q, int qp, float (*lleft)[5], float (*ldiag)[5
                 V[rr][cc] = cell_f32(dg, up, lf, qt, dt, -9.0f, NULL);
             }
             for (int s = 0; s < 5; s++) {
-                lleft[r][s] = (rr == 31) ? 0.0f : V[rr][31].v[s];
-                ldiag[r][s] = (rr == 0 || rr == 31) ? 0.0f : V[rr - 1][31].v[s];
+                lleft[r][s] = (rr == 31 && !noquirk) ? 0.0f : V[rr][31].v[s];
+                ldiag[r][s] = (rr == 0 || (rr == 31 && !noquirk)) ? 0.0f : V[rr - 1][31].v[s];
             }
         }
     }

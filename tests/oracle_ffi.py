@@ -109,3 +109,61 @@ class Oracle:
         s = C.c_float(); l = C.c_float()
         self.L.or_score_threshold(C.byref(self.o), qlen, dblen, C.byref(s), C.byref(l))
         return s.value, l.value
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("Min", "Max", "E", "H", "Lmbd", "K",
+                                          "LmbdEst", "KEst", "Evalue", "EpA")]
+
+
+def _ss_sigs(L):
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C")
+    i64 = np.ctypeslib.ndpointer(np.int64, flags="C")
+    L.or_histogram.argtypes = [fp, C.c_int, C.c_int, C.c_char_p, C.c_char_p, C.c_int, i64]
+    L.or_statistics.argtypes = [i64, C.c_int, C.c_int, C.c_float, C.c_float, C.c_float, C.c_float,
+                                C.c_float, C.POINTER(Options), C.POINTER(C.c_float)]
+    L.or_statistics.restype = Stats
+    L.or_length_adjustment.argtypes = [C.POINTER(Options), C.c_uint64, C.c_uint64, C.c_uint64,
+                                       C.POINTER(C.c_uint64)]
+    L.or_length_adjustment.restype = C.c_uint64
+    for n in ("or_lbetainc",):
+        getattr(L, n).argtypes = [C.c_double, C.c_double, C.c_double]
+        getattr(L, n).restype = C.c_double
+    for n in ("or_lgammainc_Q", "or_lgammainc_P"):
+        getattr(L, n).argtypes = [C.c_double, C.c_double]
+        getattr(L, n).restype = C.c_double
+    L.or_set_flags.argtypes = [C.c_int]
+
+
+def _oracle_histogram(self, S, qaa, daa, xuninf=1):
+    Lq, Ld = S.shape
+    cnt = np.zeros(641, np.int64)
+    self.L.or_histogram(np.ascontiguousarray(S, np.float32), Lq, Ld, qaa, daa, xuninf, cnt)
+    return cnt
+
+
+def _oracle_statistics(self, counts, Lq, Ld, qeno, deno, score, psts, searchspace):
+    nn = np.ascontiguousarray(np.ctypeslib.as_array(self.t.nnw), np.float32)
+    return self.L.or_statistics(np.ascontiguousarray(counts, np.int64), Lq, Ld, qeno, deno, score,
+                                psts, searchspace, C.byref(self.o),
+                                nn.ctypes.data_as(C.POINTER(C.c_float)))
+
+
+def _oracle_length_adjustment(self, m, n, N):
+    d = C.c_uint64()
+    ss = self.L.or_length_adjustment(C.byref(self.o), m, n, N, C.byref(d))
+    return ss, d.value
+
+
+Oracle.histogram = _oracle_histogram
+Oracle.statistics = _oracle_statistics
+Oracle.length_adjustment = _oracle_length_adjustment
+_orig_lib = lib
+
+
+def lib():  # noqa: F811
+    L = _orig_lib()
+    if not getattr(L, "_ss_ready", False):
+        _ss_sigs(L)
+        L._ss_ready = True
+    return L

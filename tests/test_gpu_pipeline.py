@@ -1,0 +1,63 @@
+"""Phase-1 finalize and statistics on the GPU vs the oracle.
+
+Finalize records (score after the correlation bonus, positive-score count,
+beginning/end coordinates) must match the FP32 oracle bit for bit; the
+statistics are FP64 on both sides and must agree to 1e-9 relative.
+"""
+import numpy as np
+import pytest
+from comer2_amd._ffi import SynthDB
+from gpu_common import oracle, processing
+
+pytestmark = pytest.mark.gpu
+
+
+def _expected(orc, q, qp, d, thld, searchspace):
+    exp = []
+    qaa = bytes(q.view.aa[q.view.dist[qp]:q.view.dist[qp] + q.view.len[qp]])
+    for dp in range(d.nprof):
+        S, _ = orc.score_f32(q.view, qp, d.view, dp)
+        m, bt, _ = orc.viterbi_f32(q.view, qp, d.view, dp, S)
+        if not (m.maxsc > thld):
+            continue
+        f = orc.finalize(S, bt, m)
+        if not (thld < f.score):
+            continue
+        daa = bytes(d.view.aa[d.view.dist[dp]:d.view.dist[dp] + d.view.len[dp]])
+        cnt = orc.histogram(S, qaa, daa, 1)
+        st = orc.statistics(cnt, q.view.len[qp], d.view.len[dp], q.view.eno[qp], d.view.eno[dp],
+                            f.score, float(f.psts), np.float32(searchspace))
+        exp.append((dp, f, st))
+    return exp
+
+
+@pytest.mark.parametrize("rmax", [4, 2])
+def test_pipeline_phase1_finalize_and_stats(rmax):
+    q = SynthDB(21, 2, 150, 0.668, 30, 400)
+    d = SynthDB(22, 80, 180, 0.668, 30, 800)
+    orc = oracle()
+    cbp = processing()
+    cbp.set_tuning(2048, rmax, 8)
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    thld = 1.0
+    for qp in range(q.nprof):
+        res = cbp.ProcessScoreMatrix(qp, scorethld=thld, logevthld=np.log(10.0))
+        ss, _ = orc.length_adjustment(q.view.len[qp], d.npos, d.nprof)
+        assert res["search_space"] == ss
+        exp = _expected(orc, q, qp, d, thld, ss)
+        assert res["npassed"] == len(exp) > 0
+        for k, (dp, f, st) in enumerate(exp):
+            assert res["pronr"][k] == dp
+            assert res["score"][k].view(np.uint32) == np.float32(f.score).view(np.uint32)
+            assert res["psts"][k] == f.psts
+            assert res["beg"][k] == ((f.begx << 16) | f.begy)
+            assert res["end"][k] == ((f.endx << 16) | f.endy)
+            for name, val in (("lmbd", st.Lmbd), ("K", st.K), ("H", st.H), ("E", st.E),
+                              ("logE", st.Evalue), ("logEpA", st.EpA), ("smin", st.Min),
+                              ("smax", st.Max)):
+                got = float(res[name][k])
+                assert abs(got - val) <= 1e-6 * max(1.0, abs(val)), (name, dp, got, val)
+        le = res["logE"]
+        assert np.all(np.diff(le[res["order"]]) >= 0)
