@@ -29,7 +29,8 @@ class Results(C.Structure):
     _fields_ = [("npassed", C.c_int32), ("nposits", C.c_int64), ("maxlen", C.c_int32),
                 ("nmap", C.c_int32), ("alndata", C.POINTER(C.c_float)),
                 ("order", C.POINTER(C.c_int32)), ("alns", C.POINTER(C.c_char)),
-                ("alnstride", C.c_int64), ("scorethld", C.c_float), ("logevthld", C.c_float),
+                ("alnstride", C.c_int64), ("map_rows", C.POINTER(C.c_int32)),
+                ("epa", C.POINTER(C.c_float)), ("scorethld", C.c_float), ("logevthld", C.c_float),
                 ("search_space", C.c_double)]
 
 
@@ -58,6 +59,9 @@ def load_synth():
     lib.c2s_free.argtypes = [C.c_void_p]
     lib.c2s_subset.restype = C.c_void_p
     lib.c2s_subset.argtypes = [C.c_void_p, C.c_int32, C.c_int32]
+    lib.c2s_plant.restype = C.c_int32
+    lib.c2s_plant.argtypes = [C.c_void_p, C.POINTER(Profiles), C.c_uint64, C.c_int32,
+                              C.c_double, C.c_double, C.c_void_p, C.c_void_p]
     return lib
 
 
@@ -72,6 +76,14 @@ class SynthDB:
         self.h = _handle if _handle is not None else SynthDB._lib.c2s_generate(
             seed, nprof, median, sigma, minlen, maxlen, 1 if sort_desc else 0, conc)
         self.view = SynthDB._lib.c2s_view(self.h).contents
+
+    def plant(self, src, seed, nplant, kmin=4.0, kmax=60.0):
+        """Plant homologous segments of the profiles of `src` (see c2s_plant)."""
+        ps = np.zeros(nplant, np.int32)
+        pp = np.zeros(nplant, np.int32)
+        n = SynthDB._lib.c2s_plant(self.h, C.byref(src.view), seed, nplant, kmin, kmax,
+                                   ps.ctypes.data_as(C.c_void_p), pp.ctypes.data_as(C.c_void_p))
+        return ps[:n], pp[:n]
 
     def subset(self, p0, n):
         return SynthDB(0, 0, 0, _handle=SynthDB._lib.c2s_subset(self.h, p0, n))

@@ -37,7 +37,7 @@ struct DevSet {
 
 struct DevQuery {
     QueryPrep hp;
-    DBuf qv, qtr, QT, qh, lleft, ldiag, qbkg, qaa;
+    DBuf qv, qtr, qtrb, QT, qh, lleft, ldiag, qbkg, qaa, mleft, mdiag;
 };
 
 struct cbp_handle {
@@ -62,6 +62,7 @@ struct cbp_handle {
     DBuf dnnw;           /* SS18 parameters on the device */
     /* per-call scratch */
     DBuf cand, cand_off, btck, fin, pass_pro, counts, statin, statout;
+    DBuf mpairs, mF, mbt, mstat, malns, malntmp, mres;
     int32_t nseg = 0;
     DBuf keys;           /* [nprof] u64 */
     DBuf bot;            /* stripe hand-off */
@@ -75,6 +76,8 @@ struct cbp_handle {
     std::vector<float> res_aln;
     std::vector<int32_t> res_order;
     std::vector<char> res_strs;
+    std::vector<float> res_epa;   /* log pair E-values before MAP reuses the slot */
+    std::vector<int> res_map;     /* alndata rows realigned, in alns order */
 };
 
 #endif

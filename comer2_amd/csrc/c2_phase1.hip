@@ -40,9 +40,11 @@ constexpr int BLK = 32;       /* columns loaded per refill */
 constexpr int F_DW = 0, F_DT = 20, F_PJ = 27, F_X = 30, F_LEN = 31, F_PID = 32, F_TOP = 33;
 constexpr int NF_BASE = 33, NF_TOP = 38;
 
+/* tops: the ring carries the stripe's top boundary; rtop: read it from the
+ * hand-off buffer (stripes after the first) instead of the zero row -1 */
 template <int R>
 __device__ __forceinline__ void load_block(const P1Args& a, float* ring, int lane, int t0,
-                                           int64_t c_begin, int64_t c_end, bool tops)
+                                           int64_t c_begin, int64_t c_end, bool tops, bool rtop)
 {
     if (lane >= BLK) return;
     const int slot = (t0 + lane) % RING;
@@ -69,7 +71,8 @@ __device__ __forceinline__ void load_block(const P1Args& a, float* ring, int lan
         ring[F_LEN * RING + slot] = __int_as_float(a.len[p]);
         ring[F_PID * RING + slot] = __int_as_float(p);
         if (tops)
-            for (int s = 0; s < 5; s++) ring[(F_TOP + s) * RING + slot] = a.bot[(size_t)s * a.npos + c];
+            for (int s = 0; s < 5; s++)
+                ring[(F_TOP + s) * RING + slot] = rtop ? a.bot[(size_t)s * a.npos + c] : 0.0f;
     } else {
         for (int k = 0; k < 20; k++) ring[(F_DW + k) * RING + slot] = 0.0f;
         for (int k = 0; k < 7; k++) ring[(F_DT + k) * RING + slot] = -32768.0f;
@@ -146,7 +149,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
             if (tops) for (int j = 0; j < 5; j++) ring[(F_TOP + j) * RING + i] = 0.0f;
         }
         wave_lds_sync();
-        load_block<R>(a, ring, lane, 0, c_begin, c_end, tops);
+        load_block<R>(a, ring, lane, 0, c_begin, c_end, tops, s > 0);
         wave_lds_sync();
 
         S5 H[R];
@@ -166,7 +169,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
 
         for (int t = 0; t < nsteps; t++) {
             if (((t + 1) % BLK) == 0) {
-                load_block<R>(a, ring, lane, t + 1, c_begin, c_end, tops);
+                load_block<R>(a, ring, lane, t + 1, c_begin, c_end, tops, s > 0);
                 wave_lds_sync();
             }
             const int slot = (t - lane + RING * 4) % RING;

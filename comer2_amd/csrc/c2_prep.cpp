@@ -19,6 +19,7 @@
 #include <cstring>
 #include <algorithm>
 #include "c2_prep.h"
+#include "c2_fmath.h"
 
 namespace {
 enum { TMM = 0, TMI, TMD, TIM, TII, TDM, TDD };
@@ -77,6 +78,55 @@ void c2_colm1(const cbp_profiles* q, int qp, float* lleft, float* ldiag)
     }
 }
 
+namespace {
+inline float mexp(float x) { return (-16.0f < x) ? c2_expf(x) : 0.0f; }
+inline float mlog(float x) { return (3.8e-11f < x) ? c2_logf(x) : -99.0f; }
+inline float lse2(float a, float b) { float m = mx(a, b); return mlog(mexp(a - m) + mexp(b - m)) + m; }
+/* forward cell of ExecMAPDP_Fwd_Unroll32x (cumapdp/CuBatchMAPDP_fwd.cu:291-372) */
+St fcell(const St& dg, const St& up, const St& lf, const float* qt, const float* dt, float sco) {
+    St n; float v1, v2, v3, v4, v5, m = 0.0f;
+    v1 = dg.s[1] + qt[0] + dt[3] + sco; m = mx(m, v1);
+    v2 = dg.s[2] + qt[3] + dt[0] + sco; m = mx(m, v2);
+    v3 = dg.s[3] + qt[5] + dt[0] + sco; m = mx(m, v3);
+    v4 = dg.s[4] + qt[0] + dt[5] + sco; m = mx(m, v4);
+    v5 = dg.s[0] + qt[0] + dt[0] + sco; m = mx(m, v5);
+    v1 = mexp(v1 - m) + mexp(v2 - m) + mexp(v3 - m) + mexp(v4 - m) + mexp(v5 - m) + mexp(-m);
+    n.s[0] = mlog(v1) + m;
+    n.s[1] = lse2(up.s[0] + qt[0] + dt[1], up.s[1] + qt[0] + dt[4]);
+    n.s[2] = lse2(lf.s[0] + qt[1] + dt[0], lf.s[2] + qt[4] + dt[0]);
+    n.s[3] = lse2(up.s[0] + qt[2], up.s[3] + qt[6]);
+    n.s[4] = lse2(lf.s[0] + dt[2], lf.s[4] + dt[6]);
+    return n;
+}
+}  // namespace
+
+/* The MAP forward pass has the phase-1 column -1 structure with the MAP
+ * kernels' initial values (MM 0, other states log 0 = -99) and out-of-range
+ * transitions and scores of -99 (cumapdp/CuBatchMAPDP_fwd.cu:210-268). */
+void c2_map_colm1(const cbp_profiles* q, int qp, float* mleft, float* mdiag)
+{
+    const int Lq = q->len[qp];
+    const St init = {{0.0f, -99.0f, -99.0f, -99.0f, -99.0f}};
+    float dt[7];
+    for (int k = 0; k < 7; k++) dt[k] = -99.0f;
+    std::vector<St> V(32 * 33);
+    for (int b = 0; 32 * b < Lq; b++)
+        for (int rr = 0; rr < 32 && 32 * b + rr < Lq; rr++) {
+            const int r = 32 * b + rr, first = 31 - rr;
+            float qt[7]; trn7(q, qp, r, qt);
+            for (int cc = first; cc < 32; cc++) {
+                const St& up = (rr > 0 && cc > first) ? V[(rr - 1) * 33 + cc] : init;
+                const St& dg = (rr > 0 && cc - 1 > first) ? V[(rr - 1) * 33 + cc - 1] : init;
+                const St& lf = (cc > first) ? V[rr * 33 + cc - 1] : init;
+                V[rr * 33 + cc] = fcell(dg, up, lf, qt, dt, -99.0f);
+            }
+            for (int s = 0; s < 5; s++) {
+                mleft[r * 5 + s] = (rr == 31) ? init.s[s] : V[rr * 33 + 31].s[s];
+                mdiag[r * 5 + s] = (rr == 0 || rr == 31) ? init.s[s] : V[(rr - 1) * 33 + 31].s[s];
+            }
+        }
+}
+
 void c2_geometry(int Lq, int rmax, int* R, int* nstripes)
 {
     if (rmax < 1) rmax = 1;
@@ -98,6 +148,7 @@ void c2_prepare_query(const cbp_profiles* q, int qp, const float* ssss, int rmax
     const int Lp = o->Lpad;
     o->qv.assign((size_t)20 * Lp, 0.0f);
     o->qtr.assign((size_t)7 * Lp, -32768.0f);
+    o->qtrb.assign((size_t)7 * Lp, -99.0f);
     o->QT.assign((size_t)30 * Lp, 0.0f);
     o->qh.assign((size_t)Lp, 0.0f);
     o->lleft.assign((size_t)5 * Lp, 0.0f);
@@ -106,10 +157,20 @@ void c2_prepare_query(const cbp_profiles* q, int qp, const float* ssss, int rmax
     for (int a = 0; a < 20; a++) o->qbkg[a] = q->bkg[(size_t)a * q->nprof + qp];
     std::vector<float> ll((size_t)5 * Lq), ld((size_t)5 * Lq);
     c2_colm1(q, qp, ll.data(), ld.data());
+    std::vector<float> ml((size_t)5 * Lq), md((size_t)5 * Lq);
+    c2_map_colm1(q, qp, ml.data(), md.data());
+    o->mleft.assign((size_t)5 * Lq, 0.0f);
+    o->mdiag.assign((size_t)5 * Lq, 0.0f);
+    for (int r = 0; r < Lq; r++)
+        for (int s = 0; s < 5; s++) {
+            o->mleft[(size_t)s * Lq + r] = ml[(size_t)r * 5 + s];
+            o->mdiag[(size_t)s * Lq + r] = md[(size_t)r * 5 + s];
+        }
     for (int r = 0; r < Lq; r++) {
         for (int a = 0; a < 20; a++) o->qv[(size_t)a * Lp + r] = q->trgfrq[(size_t)a * q->npos + q0 + r];
         float t[7]; trn7(q, qp, r, t);
         for (int k = 0; k < 7; k++) o->qtr[(size_t)k * Lp + r] = t[k];
+        for (int k = 0; k < 7; k++) o->qtrb[(size_t)k * Lp + r] = trn(q, qp, k, r + 1);
         float p[3]; int ri[3];
         for (int i = 0; i < 3; i++) {
             p[i] = q->ssprb[(size_t)i * q->npos + q0 + r];

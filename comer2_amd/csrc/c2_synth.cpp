@@ -211,6 +211,69 @@ const cbp_profiles* c2s_view(void* h) { return &((SynthDB*)h)->view; }
 const int32_t* c2s_orgidx(void* h) { return ((SynthDB*)h)->orgidx.data(); }
 void c2s_free(void* h) { delete (SynthDB*)h; }
 
+/* Plants homologous segments: nplant DB profiles (chosen by the seed) receive,
+ * over a window of w = U(0.5,1) * min(Ldb, Ls) positions, target
+ * probabilities ~ Dirichlet(k * t_src + 0.5 bg) and SS probabilities
+ * ~ Dirichlet(k * p_src + 0.2) copied from a window of a source profile
+ * (source profile chosen round-robin), with k ~ U(kmin, kmax) per planted
+ * profile; residues and SS states are re-derived (argmax).  Returns the
+ * number planted; plant_src[i] / plant_pro[i] record (source, DB profile). */
+int32_t c2s_plant(void* h, const cbp_profiles* src, uint64_t seed, int32_t nplant,
+                  double kmin, double kmax, int32_t* plant_src, int32_t* plant_pro)
+{
+    SynthDB* db = (SynthDB*)h;
+    const int32_t np = db->view.nprof;
+    if (nplant > np) nplant = np;
+    Philox g(seed, 0xABCDEF0123ull);
+    std::vector<int32_t> idx(np);
+    std::iota(idx.begin(), idx.end(), 0);
+    for (int32_t i = 0; i < nplant; i++) {       /* partial Fisher-Yates */
+        int32_t j = i + (int32_t)(g.uniform() * (np - i));
+        if (j >= np) j = np - 1;
+        std::swap(idx[i], idx[j]);
+    }
+    std::sort(idx.begin(), idx.begin() + nplant);
+    const int64_t npos = db->view.npos;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int32_t i = 0; i < nplant; i++) {
+        const int32_t p = idx[i];
+        const int32_t sp = i % src->nprof;
+        Philox r(seed, 0x100000000ull + (uint64_t)p);
+        const int32_t Ld = db->len[p], Ls = src->len[sp];
+        const int32_t m = Ld < Ls ? Ld : Ls;
+        int32_t w = (int32_t)((0.5 + 0.5 * r.uniform()) * m);
+        if (w < 1) w = 1;
+        const int32_t so = (int32_t)(r.uniform() * (Ls - w + 1));
+        const int32_t dof = (int32_t)(r.uniform() * (Ld - w + 1));
+        const double k = kmin + (kmax - kmin) * r.uniform();
+        double alpha[20], tq[20], a3[3], s3[3];
+        for (int32_t j = 0; j < w; j++) {
+            const int64_t spos = (int64_t)src->dist[sp] + so + j;
+            const int64_t pos = (int64_t)db->dist[p] + dof + j;
+            for (int a = 0; a < 20; a++)
+                alpha[a] = k * (double)src->trgfrq[(size_t)a * src->npos + spos] + 0.5 * (double)kRobinson[a];
+            dirichlet(r, alpha, 20, tq);
+            int am = 0;
+            for (int a = 0; a < 20; a++) {
+                db->trgfrq[(size_t)a * npos + pos] = (float)tq[a];
+                if (tq[a] > tq[am]) am = a;
+            }
+            db->aa[pos] = kAlpha[am];
+            for (int s = 0; s < 3; s++) a3[s] = k * (double)src->ssprb[(size_t)s * src->npos + spos] + 0.2;
+            dirichlet(r, a3, 3, s3);
+            int sm = 0;
+            for (int s = 0; s < 3; s++) {
+                db->ssprb[(size_t)s * npos + pos] = (float)s3[s];
+                if (s3[s] > s3[sm]) sm = s;
+            }
+            db->sss[pos] = (char)sm;
+        }
+        if (plant_src) plant_src[i] = sp;
+        if (plant_pro) plant_pro[i] = p;
+    }
+    return nplant;
+}
+
 /* Copies profiles [p0, p0+n) of a set into a new owned set (dist rebased). */
 void* c2s_subset(void* h, int32_t p0, int32_t n)
 {

@@ -160,14 +160,20 @@ def results_to_dict(r):
                score=a[:, _ffi.F_SCORE], pronr=u[:, _ffi.F_ORGPRONO].astype(np.int64),
                dst=u[:, _ffi.F_PRONEWDST].astype(np.int64), psts=a[:, _ffi.F_PSTVS],
                beg=u[:, _ffi.F_BEGCOORDS], end=u[:, _ffi.F_ENDCOORDS],
-               logE=a[:, _ffi.F_EVALUE], logEpA=a[:, _ffi.F_EPA],
+               logE=a[:, _ffi.F_EVALUE],
                lmbd=a[:, _ffi.F_LMBD], K=a[:, _ffi.F_K], H=a[:, _ffi.F_H], E=a[:, _ffi.F_E],
                smin=a[:, _ffi.F_MIN], smax=a[:, _ffi.F_MAX],
                lmbdest=a[:, _ffi.F_LMBDEST], kest=a[:, _ffi.F_KEST],
                idnts=a[:, _ffi.F_IDNTS], ngaps=a[:, _ffi.F_NGAPS])
+    out["epa"] = np.ctypeslib.as_array(r.epa, (n,)).copy() if n > 0 else np.zeros(0, np.float32)
+    out["alnlen"] = u[:, _ffi.F_EPA].astype(np.int64)
+    out["map_rows"] = np.zeros(0, np.int32)
+    out["alns"] = []
     if r.nmap > 0 and r.alns:
-        out["alns"] = np.ctypeslib.as_array(C.cast(r.alns, C.POINTER(C.c_uint8)),
-                                            (r.nmap, 3, r.alnstride)).copy()
+        raw = np.ctypeslib.as_array(C.cast(r.alns, C.POINTER(C.c_uint8)), (r.nmap, 3, r.alnstride))
+        out["map_rows"] = np.ctypeslib.as_array(r.map_rows, (r.nmap,)).copy()
+        out["alns"] = [tuple(bytes(raw[k, j]).split(b"\0")[0].decode() for j in range(3))
+                       for k in range(r.nmap)]
     return out
 
 

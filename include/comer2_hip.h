@@ -86,8 +86,13 @@ typedef struct {
     const float* alndata;    /* [npassed][CBP_NFIELDS] in profile order */
     const int32_t* order;    /* [npassed] indices into alndata sorted by (log E, profile index) */
     const char*  alns;       /* alignment strings: per realigned pair, 3 rows
-                                of alnstride chars (query, middle, target) */
+                                of alnstride chars (query, middle, target),
+                                NUL-terminated */
     int64_t      alnstride;
+    const int32_t* map_rows; /* [nmap] alndata row of each realigned pair */
+    const float* epa;        /* [npassed] log pair E-value from the statistics
+                                (the EPA field holds the MAP alignment length,
+                                as in the reference's output records) */
     float        scorethld, logevthld;
     double       search_space;
 } cbp_results;

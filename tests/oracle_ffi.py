@@ -167,3 +167,66 @@ def lib():  # noqa: F811
         _ss_sigs(L)
         L._ss_ready = True
     return L
+
+
+class MapRes(C.Structure):
+    _fields_ = [("logZ", C.c_float), ("maxF", C.c_float), ("maxsc", C.c_float),
+                ("x", C.c_int32), ("y", C.c_int32), ("alnlen", C.c_int32), ("idts", C.c_int32),
+                ("psts", C.c_int32), ("gaps", C.c_int32), ("begx", C.c_int32), ("begy", C.c_int32)]
+
+
+def _map_sigs(L):
+    P = C.POINTER(Profiles)
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C")
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    L.or_map_refemu.argtypes = [P, C.c_int, P, C.c_int, fp, fp, C.c_float, C.c_float, C.c_int,
+                                C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_map_refemu.restype = MapRes
+    L.or_map_direct.argtypes = [P, C.c_int, P, C.c_int, fp, fp, C.c_float, C.c_float, C.c_int,
+                                C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]
+    L.or_map_direct.restype = MapRes
+    L.or_map_f64.argtypes = [P, C.c_int, P, C.c_int, dp, dp, C.c_float, C.c_int, C.c_void_p]
+    L.or_map_f64.restype = C.c_double
+    L.or_map_dynparams.argtypes = [C.c_float, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+
+
+def _ptr(a):
+    return None if a is None else a.ctypes.data_as(C.c_void_p)
+
+
+def _oracle_map(self, q, qp, d, dp, S, M, epa, direct=True, want=True):
+    L = self.L
+    if not getattr(L, "_map_ready", False):
+        _map_sigs(L)
+        L._map_ready = True
+    Lq, Ld = q.len[qp], d.len[dp]
+    F = np.zeros((Lq, Ld), np.float32) if want else None
+    FB = np.zeros((Lq, Ld), np.float32) if want else None
+    bt = np.zeros((Lq, Ld), np.int8) if want else None
+    modin = 1 if self.o.ssswgt > 0 else 0
+    if direct:
+        aln = np.zeros((3, Lq + Ld + 1), np.uint8) if want else None
+        r = L.or_map_direct(C.byref(q), qp, C.byref(d), dp, np.ascontiguousarray(S, np.float32),
+                            np.ascontiguousarray(M, np.float32), epa, self.o.minpp, modin,
+                            _ptr(F), _ptr(FB), _ptr(bt), _ptr(aln))
+        return r, F, FB, bt, aln
+    r = L.or_map_refemu(C.byref(q), qp, C.byref(d), dp, np.ascontiguousarray(S, np.float32),
+                        np.ascontiguousarray(M, np.float32), epa, self.o.minpp, modin,
+                        _ptr(F), _ptr(FB), _ptr(bt))
+    return r, F, FB, bt, None
+
+
+def _oracle_map64(self, q, qp, d, dp, S64, M64, epa):
+    L = self.L
+    if not getattr(L, "_map_ready", False):
+        _map_sigs(L)
+        L._map_ready = True
+    post = np.zeros((q.len[qp], d.len[dp]), np.float64)
+    modin = 1 if self.o.ssswgt > 0 else 0
+    lz = L.or_map_f64(C.byref(q), qp, C.byref(d), dp, np.ascontiguousarray(S64), np.ascontiguousarray(M64),
+                      epa, modin, _ptr(post))
+    return lz, post
+
+
+Oracle.map = _oracle_map
+Oracle.map64 = _oracle_map64

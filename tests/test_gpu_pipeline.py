@@ -51,11 +51,12 @@ def test_pipeline_phase1_finalize_and_stats(rmax):
         for k, (dp, f, st) in enumerate(exp):
             assert res["pronr"][k] == dp
             assert res["score"][k].view(np.uint32) == np.float32(f.score).view(np.uint32)
-            assert res["psts"][k] == f.psts
-            assert res["beg"][k] == ((f.begx << 16) | f.begy)
-            assert res["end"][k] == ((f.endx << 16) | f.endy)
+            if k not in set(res["map_rows"].tolist()):   # realigned rows carry MAP's values
+                assert res["psts"][k] == f.psts
+                assert res["beg"][k] == ((f.begx << 16) | f.begy)
+                assert res["end"][k] == ((f.endx << 16) | f.endy)
             for name, val in (("lmbd", st.Lmbd), ("K", st.K), ("H", st.H), ("E", st.E),
-                              ("logE", st.Evalue), ("logEpA", st.EpA), ("smin", st.Min),
+                              ("logE", st.Evalue), ("epa", st.EpA), ("smin", st.Min),
                               ("smax", st.Max)):
                 got = float(res[name][k])
                 assert abs(got - val) <= 1e-6 * max(1.0, abs(val)), (name, dp, got, val)

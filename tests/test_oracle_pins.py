@@ -1,0 +1,194 @@
+"""Pins of the oracle (CPU only).
+
+Each check is chosen so that a plausible mistake in the restatement fails it:
+  - the direct Viterbi / MAP recurrences against literal emulations of the
+    reference's 32x32 oblique-block schedules (every cell, backtrack byte,
+    maximum), including a planted mistake (the column -1 quirk switched off)
+    that the emulation must catch;
+  - closed forms for the score (q = d = background), the SS table entry
+    T[9][9] = 1.0845, lambda and K for two-score distributions, and the
+    length-adjustment fixed point;
+  - SciPy for the log incomplete beta / gamma functions on the SS18 domain;
+  - FP32 mirror against FP64 for scores, DP maxima and MAP posteriors.
+"""
+import math
+import numpy as np
+import pytest
+import scipy.special as sp
+import scipy.optimize as sopt
+from comer2_amd._ffi import SynthDB
+from comer2_amd import tables
+from gpu_common import oracle
+import oracle_ffi as of
+
+
+@pytest.fixture(scope="module")
+def orc():
+    return oracle()
+
+
+@pytest.fixture(scope="module")
+def sets():
+    q = SynthDB(7, 4, 150, 0.668, 30, 400)
+    d = SynthDB(8, 24, 120, 0.668, 30, 400)
+    d.plant(q, 9, 8, 4.0, 60.0)
+    return q, d
+
+
+def test_viterbi_matches_reference_schedule(orc, sets):
+    q, d = sets
+    for qp in range(q.nprof):
+        for dp in range(d.nprof):
+            S, _ = orc.score_f32(q.view, qp, d.view, dp)
+            r1, b1, m1 = orc.viterbi_f32(q.view, qp, d.view, dp, S)
+            r2, b2, m2 = orc.refemu(q.view, qp, d.view, dp, S)
+            assert np.array_equal(m1.view(np.uint32), m2.view(np.uint32))
+            assert np.array_equal(b1, b2)
+            assert r1.maxsc == r2.maxsc
+            if (m1 == r1.maxsc).sum() == 1:
+                assert (r1.x, r1.y) == (r2.x, r2.y)
+
+
+def test_planted_mistake_is_caught(orc, sets):
+    """Without the reference's column -1 zero rows the emulation disagrees."""
+    q, d = sets
+    L = of.lib()
+    L.or_set_flags(1)
+    try:
+        diff = 0
+        for qp in range(q.nprof):
+            for dp in range(d.nprof):
+                S, _ = orc.score_f32(q.view, qp, d.view, dp)
+                _, b1, m1 = orc.viterbi_f32(q.view, qp, d.view, dp, S)
+                _, b2, m2 = orc.refemu(q.view, qp, d.view, dp, S)
+                diff += int(not np.array_equal(m1.view(np.uint32), m2.view(np.uint32)))
+        assert diff > 0
+    finally:
+        L.or_set_flags(0)
+
+
+def test_map_matches_reference_schedule(orc, sets):
+    q, d = sets
+    n_aln = 0
+    for qp in range(2):
+        for dp in range(d.nprof):
+            S, M = orc.score_f32(q.view, qp, d.view, dp)
+            for epa in (-40.0, -19.0, 3.0):
+                r1, F1, FB1, b1, _ = orc.map(q.view, qp, d.view, dp, S, M, epa, direct=False)
+                r2, F2, FB2, b2, _ = orc.map(q.view, qp, d.view, dp, S, M, epa, direct=True)
+                assert np.array_equal(F1.view(np.uint32), F2.view(np.uint32))
+                assert np.array_equal(FB1.view(np.uint32), FB2.view(np.uint32))
+                assert r1.logZ == r2.logZ and r1.maxsc == r2.maxsc
+                assert np.array_equal(b1, b2)
+                n_aln += r2.alnlen > 0
+    assert n_aln > 0
+
+
+def test_score_closed_forms(orc):
+    q = SynthDB(3, 1, 40, 0.0, 40, 40)
+    S64, M64 = orc.score_f64(q.view, 0, q.view, 0)
+    S32, M32 = orc.score_f32(q.view, 0, q.view, 0)
+    assert np.allclose(S32, S64, rtol=0, atol=2e-6)
+    # q = d = background: S = log(sum bg) - 0.002 with no SS term
+    bg = np.ctypeslib.as_array(q.view.bkg, (20,)).astype(np.float64)
+    o = oracle(dict(ssswgt=0.0))
+    # build a one-position profile set whose target frequencies equal its background
+    tf = np.ctypeslib.as_array(q.view.trgfrq, (20, q.npos))
+    save = tf[:, 0].copy()
+    tf[:, 0] = bg.astype(np.float32)
+    try:
+        S, _ = o.score_f64(q.view, 0, q.view, 0)
+        assert abs(S[0, 0] - (math.log(np.sum(bg.astype(np.float32).astype(np.float64) ** 2 /
+                                       bg.astype(np.float32).astype(np.float64))) - np.float32(0.002))) < 1e-12
+    finally:
+        tf[:, 0] = save
+
+
+def test_ss_table_entry():
+    t = tables.load_ssss()
+    assert t[9, 9] == np.float32(1.0845)
+    assert t[0, 0] == np.float32(0.375) and t[29, 29] == np.float32(1.0836)
+    assert np.array_equal(t, t.T)
+
+
+def test_lambda_K_two_scores(orc):
+    # scores -1 (prob p) and +1 (prob q): lambda = ln(p/q), K = (p-q)^2/p
+    for p in (0.6, 0.7, 0.85):
+        qq = 1 - p
+        cnt = np.zeros(641, np.int64)
+        cnt[-32 + 320] = int(round(p * 1e6))
+        cnt[32 + 320] = int(round(qq * 1e6))
+        st = orc.statistics(cnt, 100, 100, 5.0, 5.0, 10.0, 3.0, 1e7)
+        assert abs(st.Lmbd - math.log(p / qq)) < 2e-3 * math.log(p / qq)   # root accuracy 1e-4 on lambda/32
+        assert abs(st.K - (p - qq) ** 2 / p) < 1e-12
+        assert st.Min == -1 and st.Max == 1
+        assert abs(st.E - (-p + qq)) < 1e-9
+
+
+def test_lambda_K_general_against_brute_force(orc):
+    # a three-score distribution {-2,-1,+1}: lambda solves sum p e^{l s} = 1
+    pr = {-2: 0.3, -1: 0.45, 1: 0.25}
+    cnt = np.zeros(641, np.int64)
+    for s, p in pr.items():
+        cnt[32 * s + 320] = int(round(p * 1e6))
+    st = orc.statistics(cnt, 100, 100, 5.0, 5.0, 10.0, 3.0, 1e7)
+    lam = sopt.brentq(lambda l: sum(p * math.exp(l * s) for s, p in pr.items()) - 1, 1e-3, 10)
+    assert abs(st.Lmbd - lam) < 3e-3 * lam
+    H = sum(p * math.exp(lam * s) * lam * s for s, p in pr.items())
+    assert abs(st.H - H) < 5e-3 * H
+
+
+def test_special_functions_against_scipy():
+    L = of.lib()
+    for a in np.linspace(1.0, 300, 40):
+        for x in (0.2, 0.5, 0.757213):
+            ref = math.log(sp.betainc(a, 0.1, x))
+            assert abs(L.or_lbetainc(a, 0.1, x) - ref) <= 1e-12 * max(1.0, abs(ref))
+    for a in (0.5, 1.346206, 3.0):
+        for x in (0.01, 0.5, 2.0, 7.0, 30.0, 100.0):
+            ref = math.log(sp.gammaincc(a, x))
+            assert abs(L.or_lgammainc_Q(a, x) - ref) <= 1e-12 * max(1.0, abs(ref))
+
+
+def test_length_adjustment_fixed_point(orc):
+    o = tables.default_options()
+    lam, K, a, b = o["gap_lambda"], o["gap_K"], o["gap_alpha"], o["gap_beta"]
+    # the search space must satisfy K(m-l)(n-Nl) >= max(m,n); when even l=0
+    # violates it the adjustment is zero and the space is m*n
+    for m, n, N in ((300, 50_000_000, 200_000), (150, 450_000, 2000)):
+        assert m * n - max(m, n) / K < 0
+        assert orc.length_adjustment(m, n, N) == (m * n, 0)
+    for m, n, N in ((1000, 10_000_000, 40_000), (800, 60_000_000, 200_000),
+                    (2000, 1_000_000, 5000), (1500, 300_000_000, 1_000_000)):
+        assert m * n - max(m, n) / K > 0
+        ss, delta = orc.length_adjustment(m, n, N)
+        assert delta > 0
+        # the reported space belongs to the (possibly fractional) converged
+        # length l with floor(l) == delta: recover l from (m-l)(n-Nl) = ss
+        qa, qb, qc = N, -(n + m * N), m * n - ss
+        l = (-qb - math.sqrt(qb * qb - 4 * qa * qc)) / (2 * qa)
+        assert delta - 1e-3 <= l < delta + 1 + 1e-3
+        ell = a / lam * (math.log(K) + math.log(ss)) + b
+        assert l <= ell + 1e-3 and ell - l <= 1.0 + 1e-3
+
+
+def test_fp32_against_fp64(orc, sets):
+    q, d = sets
+    worst_s = worst_v = worst_p = 0.0
+    for dp in range(d.nprof):
+        S32, M32 = orc.score_f32(q.view, 0, d.view, dp)
+        S64, M64 = orc.score_f64(q.view, 0, d.view, dp)
+        worst_s = max(worst_s, float(np.max(np.abs(S32 - S64))))
+        r, _, _ = orc.viterbi_f32(q.view, 0, d.view, dp, S32)
+        v64 = orc.viterbi_f64(q.view, 0, d.view, dp, S64)
+        if v64 > 1.0:
+            worst_v = max(worst_v, abs(r.maxsc - v64) / v64)
+        r2, F, FB, _, _ = orc.map(q.view, 0, d.view, dp, S32, M32, -30.0)
+        lz, post = orc.map64(q.view, 0, d.view, dp, S64, M64, -30.0)
+        p32 = np.exp(FB.astype(np.float64) - r2.logZ)
+        big = post > 1e-3
+        if big.any():
+            worst_p = max(worst_p, float(np.max(np.abs(p32[big] - post[big]) / post[big])))
+    assert worst_s < 5e-6          # absolute, scores are O(1)
+    assert worst_v < 1e-4          # relative, DP maxima
+    assert worst_p < 1e-4          # relative, posteriors above 1e-3
