@@ -1,0 +1,208 @@
+#!/usr/bin/env python3
+"""Search throughput of the COMER2 libmycu pipeline on one MI355X.
+
+Workload (plain method at scale): one synthetic query profile of length 300
+searched against 200,000 synthetic DB profiles (lengths log-normal, median
+200, sigma 0.668, clipped to 30-3000, about 50M positions, sorted longest
+first), with homologous segments of the query planted into 4,000 DB profiles
+so that the statistics and MAP stages have real hits; MAP realignment is
+capped to the best 2,000 pairs by (log E, profile index).  One step is one
+full pipeline pass for the query: fused score + phase-1 Viterbi over every DB
+cell, backtrack recomputation + finalize for the candidates, FP64 statistics
+and E-values, MAP forward/backward/DP/backtrace for the selected pairs, and
+the results back on the host.  The DB and the query preparation are resident
+in HBM before the timed region; every step ends with the results copied to
+the host, so the step boundary is a full device synchronisation.
+
+value = search GCUPS = steps * Lq * DB positions / timed seconds.
+
+The roofline object describes the dominant kernel (fused score + Viterbi):
+algorithmic FP32 operations per cell (87, counted from the reference's
+recurrences) times the cells of one launch, over the launch's average
+duration measured with HIP events on the library's stream; peak is MI355X's
+published FP32 rate.  traffic is read from the rocprofv3 counter summary
+committed under profiles/ when present.
+
+The CPU baseline runs the oracle's whole-pipeline restatement on a strided
+sample of the same DB (OpenMP over profiles) for a bounded time.
+
+  python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--cpu-seconds 15]
+  python bench.py --copy-calib BYTES     one plain device copy (counter scale)
+"""
+import argparse
+import json
+import os
+import re
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+FLOP_PER_CELL = 87          # 41 score + 10 SS term + 36 Viterbi (add/max/compare)
+PEAK_FP32_TFLOPS = 157.3    # MI355X published FP32 peak
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "phase1_search_c1_counters.md")
+
+DB_SEED, QUERY_SEED, PLANT_SEED = 2, 1002, 3002
+N_PROFILES, MEDIAN, SIGMA, MINLEN, MAXLEN = 200_000, 200, 0.668, 30, 3000
+QUERY_LEN, N_PLANT, PLANT_KMIN, PLANT_KMAX = 300, 4000, 3.0, 60.0
+MAP_CAP = 2000
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def copy_calib(nbytes):
+    import torch
+    x = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calib_bytes": nbytes}))
+
+
+def read_traffic():
+    """HBM bytes per launch of the phase-1 kernel from the counter summary."""
+    try:
+        with open(TRAFFIC_FILE) as f:
+            txt = f.read()
+    except OSError:
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", txt)
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(q, d, orc, thld, logev, sspace, seconds, nthreads):
+    import numpy as np
+    # strided sample keeps the length distribution of the sorted DB
+    order = np.arange(d.nprof, dtype=np.int32)
+    stride = 7
+    sample = np.concatenate([order[k::stride] for k in range(stride)])
+    cells = mcells = npass = nmap = 0
+    done = 0
+    t0 = time.perf_counter()
+    while done < sample.size and time.perf_counter() - t0 < seconds:
+        batch = sample[done:done + 1024]
+        r = orc.search(q.view, 0, d.view, batch, thld, logev, float(sspace), nthreads)
+        cells += r.cells; mcells += r.map_cells; npass += r.npassed; nmap += r.nmap
+        done += batch.size
+    dt = time.perf_counter() - t0
+    return dict(value=cells / dt / 1e9, unit="GCUPS", cores=nthreads, kind="port",
+                sample="%d of %d DB profiles (stride-%d interleave), %d cells, %d passed, "
+                       "%d realigned, %.1f s; oracle FP32 restatement, OpenMP"
+                       % (done, d.nprof, stride, cells, npass, nmap, dt))
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--nprof", type=int, default=N_PROFILES)
+    ap.add_argument("--copy-calib", type=int, default=0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build measures one GPU; the search shards over DB profiles "
+              "across GPUs by design but no multi-GPU path is built (--gpus %d refused)" % args.gpus)
+        sys.exit(2)
+    if args.copy_calib:
+        copy_calib(args.copy_calib)
+        return
+
+    from comer2_amd._ffi import SynthDB
+    from comer2_amd.host import CuBatchProcessing
+
+    t = time.perf_counter()
+    d = SynthDB(DB_SEED, args.nprof, MEDIAN, SIGMA, MINLEN, MAXLEN)
+    q = SynthDB(QUERY_SEED, 1, QUERY_LEN, 0.0, QUERY_LEN, QUERY_LEN)
+    d.plant(q, PLANT_SEED, N_PLANT if args.nprof == N_PROFILES else max(1, args.nprof // 50),
+            PLANT_KMIN, PLANT_KMAX)
+    log("bench: generated %d profiles, %d positions in %.1f s" % (d.nprof, d.npos, time.perf_counter() - t))
+
+    cbp = CuBatchProcessing(0, 0, 0, options=dict(max_map_pairs=MAP_CAP))
+    t = time.perf_counter()
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    thld, logev = cbp.score_threshold(QUERY_LEN, d.npos)
+    log("bench: uploaded in %.1f s; score threshold %.3f, log E threshold %.3f"
+        % (time.perf_counter() - t, thld, logev))
+
+    for i in range(args.warmup):
+        res = cbp.ProcessScoreMatrix(0, scorethld=thld, logevthld=logev)
+        log("bench: warmup %d done (%d passed, %d realigned)" % (i, res["npassed"], res["nmap"]))
+
+    kms = []
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        res = cbp.ProcessScoreMatrix(0, scorethld=thld, logevthld=logev)
+        ms, nl = cbp.last_kernel_ms()
+        kms.append(ms / max(nl, 1))
+    elapsed = time.perf_counter() - t0
+    log("bench: %d steps in %.3f s" % (args.steps, elapsed))
+
+    cells = QUERY_LEN * d.npos
+    gcups = args.steps * cells / elapsed / 1e9
+    kavg = sum(kms) / len(kms)
+    achieved = cells * FLOP_PER_CELL / (kavg * 1e-3) / 1e12
+    alnlen = res["alnlen"]
+    dlen = d.lengths()
+    map_cells = int(sum(QUERY_LEN * int(dlen[res["pronr"][r]]) for r in res["map_rows"]))
+
+    out = {
+        "metric": "search GCUPS at 1 GPU (plain per-query pipeline: score, Viterbi, statistics, MAP)",
+        "value": round(gcups, 3),
+        "unit": "GCUPS",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator; homologs planted)",
+        "config": {
+            "workload": "1 query (L=300) vs %d synthetic DB profiles (log-normal median %d, "
+                        "sigma %.3f, clip %d-%d), full pipeline, MAP capped at %d pairs"
+                        % (d.nprof, MEDIAN, SIGMA, MINLEN, MAXLEN, MAP_CAP),
+            "db_profiles": d.nprof, "db_positions": int(d.npos), "query_len": QUERY_LEN,
+            "seeds": {"db": DB_SEED, "query": QUERY_SEED, "plant": PLANT_SEED},
+            "planted": N_PLANT, "score_threshold": round(thld, 4), "log_e_threshold": round(logev, 4),
+            "passed": int(res["npassed"]), "realigned": int(res["nmap"]),
+            "aligned": int((alnlen[res["map_rows"]] > 0).sum()) if res["nmap"] else 0,
+            "map_cells": map_cells, "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "k_phase1 (fused score + Viterbi)",
+            "bound": "valu",
+            "achieved": round(achieved, 3),
+            "peak": PEAK_FP32_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
+            "traffic": read_traffic(),
+            "flop_per_cell": FLOP_PER_CELL,
+            "cells_per_launch": cells,
+            "avg_launch_ms": round(kavg, 4),
+        },
+        "cpu_baseline": None,
+    }
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "oracle"))
+        from oracle_ffi import Oracle
+        from comer2_amd import tables
+        orc = Oracle(tables.load_ssss(), tables.load_nn_weights(),
+                     dict(tables.default_options(), max_map_pairs=MAP_CAP))
+        nthreads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))))
+        out["cpu_baseline"] = cpu_baseline(q, d, orc, thld, logev, res["search_space"],
+                                           args.cpu_seconds, nthreads)
+        log("bench: cpu baseline %.4f GCUPS on %d threads" % (out["cpu_baseline"]["value"], nthreads))
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

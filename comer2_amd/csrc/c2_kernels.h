@@ -94,5 +94,7 @@ hipError_t c2_launch_stats(const StatParams& sp, const float* nnw /*device [93]*
                            const int32_t* counts, const StatIn* in, int32_t n,
                            double* out /* [n][10] */, hipStream_t stream);
 hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream_t stream);
+/* S and mod of query vs chunk profile p (Ld its length), row-major [Lq][Ld] */
+hipError_t c2_launch_score_block(const ScoreCtx& c, int p, int Ld, float* S, float* mod, hipStream_t stream);
 
 #endif

@@ -603,3 +603,60 @@ extern "C" int cbp_process(cbp_handle* h, int32_t qidx, float scorethld, float l
     if (!h || !out) return CBP_ERR_ARG;
     return c2_process_one(h, qidx, scorethld, logevthld, out);
 }
+
+extern "C" int cbp_process_batch(cbp_handle* h, const int32_t* qidx, int32_t nq, cbp_results* outs)
+{
+    if (!h || !qidx || !outs || nq < 0) return CBP_ERR_ARG;
+    h->batch.clear();
+    h->batch.resize((size_t)nq);
+    if (!h->have_opt || !h->have_queries || !h->have_chunk)
+        return fail(h, CBP_ERR_STATE, "options, queries and chunk must be uploaded first");
+    const uint64_t dblen = h->dblen ? h->dblen : (uint64_t)h->chunk.v.npos;
+    for (int32_t i = 0; i < nq; i++) {
+        if (qidx[i] < 0 || qidx[i] >= h->queries.v.nprof) return fail(h, CBP_ERR_ARG, "query index out of range");
+        /* per-query thresholds as JobDispatcher::CalculateScoreThreshold sets them */
+        float st = 0.0f, lt = 0.0f;
+        int rc = cbp_score_threshold(&h->opt, (uint64_t)h->queries.v.len[qidx[i]], dblen, &st, &lt);
+        if (rc) return fail(h, rc, "score threshold");
+        rc = c2_process_one(h, qidx[i], st, lt, &outs[i]);
+        if (rc) return rc;
+        /* move the per-query results out of the handle's single-call buffers */
+        BatchStore& b = h->batch[(size_t)i];
+        b.aln.swap(h->res_aln); b.order.swap(h->res_order); b.strs.swap(h->res_strs);
+        b.epa.swap(h->res_epa); b.map.swap(h->res_map);
+        cbp_results& o = outs[i];
+        o.alndata = o.npassed ? b.aln.data() : nullptr;
+        o.order = o.npassed ? b.order.data() : nullptr;
+        o.epa = o.npassed ? b.epa.data() : nullptr;
+        o.alns = o.nmap ? b.strs.data() : nullptr;
+        o.map_rows = o.nmap ? b.map.data() : nullptr;
+    }
+    return CBP_OK;
+}
+
+extern "C" int cbp_score_block(cbp_handle* h, int32_t qidx, int32_t pronr, float* S, float* mod)
+{
+    if (!h || !S) return CBP_ERR_ARG;
+    if (!h->have_opt || !h->have_tables || !h->have_queries || !h->have_chunk)
+        return fail(h, CBP_ERR_STATE, "options, tables, queries and chunk must be uploaded first");
+    if (qidx < 0 || qidx >= h->queries.v.nprof) return fail(h, CBP_ERR_ARG, "query index out of range");
+    if (pronr < 0 || pronr >= h->dchunk.nprof) return fail(h, CBP_ERR_ARG, "profile index out of range");
+    (void)hipSetDevice(h->device);
+    int rc;
+    DevQuery* dq = c2_get_query(h, qidx, &rc);
+    if (!dq) return rc;
+    const int Ld = h->chunk.v.len[pronr];
+    const size_t n = (size_t)dq->hp.Lq * Ld;
+    DBuf dS, dM;
+    if ((rc = dalloc(h, dS, sizeof(float) * n))) return rc;
+    if (mod && (rc = dalloc(h, dM, sizeof(float) * n))) { h->used -= dS.n; return rc; }
+    ScoreCtx sc;
+    fill_scorectx(h, dq, &sc);
+    hipError_t e = c2_launch_score_block(sc, pronr, Ld, dS.as<float>(), mod ? dM.as<float>() : nullptr, h->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(S, dS.p, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess && mod) e = hipMemcpyAsync(mod, dM.p, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
+    h->used -= dS.n + dM.n;
+    if (e != hipSuccess) return fail(h, CBP_ERR_HIP, std::string("cbp_score_block: ") + hipGetErrorString(e));
+    return CBP_OK;
+}

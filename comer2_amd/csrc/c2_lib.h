@@ -40,6 +40,14 @@ struct DevQuery {
     DBuf qv, qtr, qtrb, QT, qh, lleft, ldiag, qbkg, qaa, mleft, mdiag;
 };
 
+/* per-query results of one cbp_process_batch call */
+struct BatchStore {
+    std::vector<float> aln, epa;
+    std::vector<int32_t> order;
+    std::vector<char> strs;
+    std::vector<int> map;
+};
+
 struct cbp_handle {
     int device = 0;
     size_t budget = 0, used = 0;
@@ -68,7 +76,7 @@ struct cbp_handle {
     DBuf bot;            /* stripe hand-off */
     uint64_t dblen = 0, nseqs = 0;
     int seg_cols = 4096; /* tuning: target columns per segment */
-    int rmax = 4;        /* tuning: max rows per lane */
+    int rmax = 5;        /* tuning: max rows per lane */
     int nw = 8;          /* tuning: waves per workgroup */
     double last_ms = 0.0;
     int32_t last_nlaunch = 0;
@@ -78,6 +86,7 @@ struct cbp_handle {
     std::vector<char> res_strs;
     std::vector<float> res_epa;   /* log pair E-values before MAP reuses the slot */
     std::vector<int> res_map;     /* alndata rows realigned, in alns order */
+    std::vector<BatchStore> batch; /* results of the last cbp_process_batch */
 };
 
 #endif

@@ -19,40 +19,14 @@
  * kernels (cuss/CuBatchSS_dynplm.cu:193-674, cuss/CuBatchSS_{lambda,entropy,K,
  * SS18,NN}.cuh, cusf/*): one thread per pair, FP64 throughout.
  */
-#include "c2_dev.h"
-#include "c2_kernels.h"
+#include "c2_score.h"
 
 namespace {
 
-/* score-matrix cell (y, col) of query vs profile p; the operation order is
- * the one of the phase-1 kernel (c2_phase1.hip) */
 __device__ __forceinline__ float score_at(const ScoreCtx& c, int y, int64_t col, int p)
 {
-    float acc = 0.0f;
-#pragma unroll
-    for (int k = 0; k < 20; k++) {
-        const float w = 1.0f / (0.5f * (c.qbkg[k] + c.bkg[(size_t)k * c.nprof + p]));
-        const float dw = c.trgfrq[(size_t)k * c.npos + col] * w;
-        const float qk = c.qv[(size_t)k * c.Lpad + y];
-        acc = (k == 0) ? qk * dw : fmaf(qk, dw, acc);
-    }
-    float sc = c2_logf_clamped(acc) - c.shift;
-    if (c.ssswgt > 0.0f) {
-        float pj[3]; int co[3];
-#pragma unroll
-        for (int j = 0; j < 3; j++) {
-            pj[j] = c.ssprb[(size_t)j * c.npos + col];
-            co[j] = 10 * j + (int)(pj[j] * 9.999f);
-        }
-        const float dh = (pj[2] + 0.1f) * 0.1111111f;
-        float ss = c.QT[(size_t)co[0] * c.Lpad + y] * pj[0];
-        ss = fmaf(c.QT[(size_t)co[1] * c.Lpad + y], pj[1], ss);
-        ss = fmaf(c.QT[(size_t)co[2] * c.Lpad + y], pj[2], ss);
-        ss = fmaf(c.qh[y], dh, ss);
-        ss = (ss - 0.05f) * c.ssswgt;
-        sc = sc + ss;
-    }
-    return sc;
+    float ss;
+    return c2_score_cell(c, y, col, p, &ss);
 }
 
 __global__ void k_finalize(ScoreCtx c, const int32_t* cand, int32_t ncand,
@@ -502,6 +476,19 @@ __global__ __launch_bounds__(64) void k_stats(StatParams sp, const float* nnw, c
     o[8] = logexp + log(ss / l1l2);
 }
 
+/* S and its SS component for one pair, row-major [Lq][Ldb] */
+__global__ __launch_bounds__(256) void k_score_block(ScoreCtx c, int p, float* S, float* mod)
+{
+    const int Ld = c.len[p];
+    const int64_t n = (int64_t)c.Lq * Ld;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int y = (int)(i / Ld), x = (int)(i - (int64_t)y * Ld);
+        float ss;
+        S[i] = c2_score_cell(c, y, (int64_t)c.dist[p] + x, p, &ss);
+        if (mod) mod[i] = ss;
+    }
+}
+
 }  // namespace
 
 hipError_t c2_launch_finalize(const ScoreCtx& c, const int32_t* cand, int32_t ncand,
@@ -526,5 +513,15 @@ hipError_t c2_launch_stats(const StatParams& sp, const float* nnw, const int32_t
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_stats, dim3((n + 63) / 64), dim3(64), 0, stream, sp, nnw, counts, in, n, out);
+    return hipGetLastError();
+}
+
+hipError_t c2_launch_score_block(const ScoreCtx& c, int p, int Ld, float* S, float* mod, hipStream_t stream)
+{
+    const int64_t n = (int64_t)c.Lq * Ld;
+    if (n <= 0) return hipSuccess;
+    int64_t g = (n + 255) / 256;
+    if (g > 65536) g = 65536;
+    hipLaunchKernelGGL(k_score_block, dim3((unsigned)g), dim3(256), 0, stream, c, p, S, mod);
     return hipGetLastError();
 }

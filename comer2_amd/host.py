@@ -43,9 +43,9 @@ def lib():
     L.cbp_phase1_keys.argtypes = [H, C.c_int32, np.ctypeslib.ndpointer(np.uint64, flags="C")]
     L.cbp_last_kernel_ms.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.cbp_set_tuning.argtypes = [H, C.c_int32, C.c_int32, C.c_int32]
-    for name in ("cbp_process",):
-        if hasattr(L, name):
-            getattr(L, name).argtypes = [H, C.c_int32, C.c_float, C.c_float, C.POINTER(Results)]
+    L.cbp_process.argtypes = [H, C.c_int32, C.c_float, C.c_float, C.POINTER(Results)]
+    L.cbp_process_batch.argtypes = [H, C.c_void_p, C.c_int32, C.POINTER(Results)]
+    L.cbp_score_block.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
     _lib = L
     return L
 
@@ -136,12 +136,29 @@ class CuBatchProcessing:
         self._check(self.L.cbp_process(self.h, qidx, scorethld, logevthld, C.byref(r)))
         return results_to_dict(r)
 
+    def ProcessBatch(self, qidxs):
+        """All listed queries against the uploaded chunk (per-query thresholds
+        from the options' EVAL); returns one result dict per query."""
+        q = np.ascontiguousarray(qidxs, np.int32)
+        outs = (Results * max(1, q.size))()
+        self._check(self.L.cbp_process_batch(self.h, q.ctypes.data_as(C.c_void_p), q.size, outs))
+        return [results_to_dict(outs[i]) for i in range(q.size)]
+
+    def score_block(self, qidx, pronr, want_mod=True):
+        """FP32 score matrix (and SS component) of one pair, [Lq][Ldb]."""
+        Lq = int(self._queries.lengths()[qidx])
+        Ld = int(self._chunk.lengths()[pronr])
+        S = np.empty((Lq, Ld), np.float32)
+        M = np.empty((Lq, Ld), np.float32) if want_mod else None
+        self._check(self.L.cbp_score_block(self.h, qidx, pronr, _fptr(S), _fptr(M) if want_mod else None))
+        return S, M
+
     def last_kernel_ms(self):
         ms, n = C.c_double(), C.c_int32()
         self._check(self.L.cbp_last_kernel_ms(self.h, C.byref(ms), C.byref(n)))
         return ms.value, n.value
 
-    def set_tuning(self, seg_cols=4096, rmax=4, nw=8):
+    def set_tuning(self, seg_cols=4096, rmax=5, nw=8):
         self._check(self.L.cbp_set_tuning(self.h, seg_cols, rmax, nw))
 
 
@@ -154,7 +171,7 @@ def results_to_dict(r):
         a = np.zeros((0, _ffi.NFIELDS), np.float32)
         order = np.zeros(0, np.int32)
     u = a.view(np.uint32)
-    out = dict(npassed=n, nposits=r.nposits, maxlen=r.maxlen, nmap=r.nmap,
+    out = SearchResults(npassed=n, nposits=r.nposits, maxlen=r.maxlen, nmap=r.nmap,
                scorethld=r.scorethld, logevthld=r.logevthld, search_space=r.search_space,
                alndata=a, order=order,
                score=a[:, _ffi.F_SCORE], pronr=u[:, _ffi.F_ORGPRONO].astype(np.int64),
@@ -168,13 +185,26 @@ def results_to_dict(r):
     out["epa"] = np.ctypeslib.as_array(r.epa, (n,)).copy() if n > 0 else np.zeros(0, np.float32)
     out["alnlen"] = u[:, _ffi.F_EPA].astype(np.int64)
     out["map_rows"] = np.zeros(0, np.int32)
-    out["alns"] = []
+    out["aln_raw"] = np.zeros((0, 3, 1), np.uint8)
     if r.nmap > 0 and r.alns:
-        raw = np.ctypeslib.as_array(C.cast(r.alns, C.POINTER(C.c_uint8)), (r.nmap, 3, r.alnstride))
+        out["aln_raw"] = np.ctypeslib.as_array(C.cast(r.alns, C.POINTER(C.c_uint8)),
+                                               (r.nmap, 3, r.alnstride)).copy()
         out["map_rows"] = np.ctypeslib.as_array(r.map_rows, (r.nmap,)).copy()
-        out["alns"] = [tuple(bytes(raw[k, j]).split(b"\0")[0].decode() for j in range(3))
-                       for k in range(r.nmap)]
     return out
+
+
+class SearchResults(dict):
+    """Result dict of one query; 'alns' (query, middle, target strings per
+    realigned pair) is decoded from the raw NUL-terminated rows on first use."""
+
+    def __missing__(self, key):
+        if key != "alns":
+            raise KeyError(key)
+        raw = self["aln_raw"]
+        val = [tuple(bytes(raw[k, j]).split(b"\0")[0].decode() for j in range(3))
+               for k in range(raw.shape[0])]
+        self["alns"] = val
+        return val
 
 
 def decode_keys(keys):

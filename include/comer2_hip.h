@@ -99,20 +99,23 @@ typedef struct {
 
 typedef struct cbp_handle cbp_handle;
 
-/* lifecycle (replaces CuDeviceMemory construction, CuDeviceMemory.cu:87-140) */
+/* lifecycle (replaces CuDeviceMemory construction, cupro/CuDeviceMemory.cu:85-146) */
 int  cbp_create(int device, size_t mem_budget, int area, cbp_handle** out);
 int  cbp_destroy(cbp_handle* h);
 const char* cbp_last_error(const cbp_handle* h);
 
-/* options + tables (CuDeviceMemory::CacheCompleteData, CuDeviceMemory.cu:583-640):
+/* options + tables (CuDeviceMemory::CacheCompleteData and its CacheSSENNWeights /
+ * CacheSSSScores, cupro/CuDeviceMemory.cu:170-209):
  * ssss900: the SS score table expanded to 30x30 (src/env/ssss);
  * nnw: SS18 network weights + adjustment parameters (93 floats,
  * CuDeviceMemory.cu:40-80 layout CuBatchProcessing_com.cuh:44-94) */
 int  cbp_set_options(cbp_handle* h, const cbp_options* opt);
 int  cbp_upload_tables(cbp_handle* h, const float* ssss900, const float* nnw, int32_t nnw_count);
 
-/* data (CuDeviceMemory::TransferCPMDataDevice, CuDeviceMemory.cu:1036-1142;
- * CuBatchProcessing::TransferCPMData) */
+/* data: queries (CuDeviceMemory::CacheData / CacheDataDevice,
+ * cupro/CuDeviceMemory.cu:234-250, 1105-1141); DB chunk
+ * (CuBatchProcessing::TransferCPMData -> CuDeviceMemory::TransferCPMData /
+ * CopyCPMDataToDevice, cupro/CuDeviceMemory.cu:251, 1036-1104) */
 int  cbp_upload_queries(cbp_handle* h, const cbp_profiles* q);
 int  cbp_upload_chunk(cbp_handle* h, const cbp_profiles* db);
 /* whole-database size used for thresholds and E-values
@@ -130,7 +133,11 @@ int  cbp_score_threshold(const cbp_options* opt, uint64_t qlen, uint64_t dblen,
 int  cbp_process(cbp_handle* h, int32_t qidx, float scorethld, float logevthld,
                  cbp_results* out);
 
-/* all listed queries against the uploaded chunk in one device run */
+/* all listed queries against the uploaded chunk; thresholds per query from
+ * cbp_score_threshold with the options' EVAL and the whole-database size
+ * (the reference's DevCommThread loop over a query batch,
+ * cualn/DevCommThread.cpp:296-356, calling ProcessScoreMatrix per query);
+ * outs[i] stays valid until the next cbp_process* call on the handle */
 int  cbp_process_batch(cbp_handle* h, const int32_t* qidx, int32_t nq,
                        cbp_results* outs /* [nq] */);
 
@@ -142,8 +149,16 @@ int  cbp_process_batch(cbp_handle* h, const int32_t* qidx, int32_t nq,
 int  cbp_phase1_keys(cbp_handle* h, int32_t qidx, uint64_t* keys /* [nprof] */);
 
 /* FP32 score matrix S and its SS component for query qidx vs chunk profile
- * pronr, row-major [qlen][dblen] (CalcSM_Init + CalcSM_SSSS) */
+ * pronr, row-major [qlen][dblen] in host memory, mod may be NULL
+ * (CalcSM_Init_SMEMUnroll2x + CalcSM_SSSS_SMEMUnroll2x,
+ * cusco/CuBatchScoreMatrix.cu:118-377) */
 int  cbp_score_block(cbp_handle* h, int32_t qidx, int32_t pronr, float* S, float* mod);
+
+/* tuning of the phase-1 launch geometry: target DB columns per wave segment
+ * (>= 64), maximum query rows per lane (1..5) and waves per workgroup
+ * (1, 2, 4, 8); no reference counterpart (the reference fixes 32x32 blocks,
+ * cudp/CuBatchDP_com.h:16-30) */
+int  cbp_set_tuning(cbp_handle* h, int32_t seg_cols, int32_t rmax, int32_t nw);
 
 /* device-side elapsed time of the last cbp_phase1_keys / cbp_process call's
  * dominant kernel (ms, HIP events on its stream) and the number of launches */

@@ -146,3 +146,22 @@ double or_map_f64(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* --- whole pipeline over a profile list (CPU baseline, coarse checker) --- */
+typedef struct {
+    int64_t cells;       /* sum of Lq*Ldb over the listed profiles */
+    int64_t map_cells;   /* sum of Lq*Ldb over the realigned pairs */
+    int32_t npassed;     /* pairs passing phase 1 */
+    int32_t nmap;        /* pairs realigned */
+    double  checksum;    /* sum of log E-values and alignment lengths */
+} or_search_out;
+or_search_out or_search(const cbp_profiles* q, int qp, const cbp_profiles* d,
+                        const int32_t* pros, int32_t npros, const or_tables* t,
+                        const cbp_options* o, float scorethld, float logevthld,
+                        float searchspace, int nthreads);
+#ifdef __cplusplus
+}
+#endif

@@ -1,4 +1,8 @@
-"""ctypes bindings to the oracle (oracle/liboracle.so) -- test infrastructure."""
+"""ctypes bindings to the oracle (oracle/liboracle.so).
+
+Test infrastructure: imported only by tests/, the smoke check and the
+benchmark's CPU-baseline leg, never by the product package.
+"""
 import ctypes as C
 import os
 import numpy as np
@@ -230,3 +234,28 @@ def _oracle_map64(self, q, qp, d, dp, S64, M64, epa):
 
 Oracle.map = _oracle_map
 Oracle.map64 = _oracle_map64
+
+
+class SearchOut(C.Structure):
+    _fields_ = [("cells", C.c_int64), ("map_cells", C.c_int64), ("npassed", C.c_int32),
+                ("nmap", C.c_int32), ("checksum", C.c_double)]
+
+
+def _oracle_search(self, q, qp, d, pros, scorethld, logevthld, searchspace, nthreads):
+    """Whole pipeline for query qp against the listed DB profiles (OpenMP)."""
+    L = self.L
+    if not getattr(L, "_search_ready", False):
+        _ss_sigs(L)
+        L._ss_ready = True
+        L.or_search.argtypes = [C.POINTER(Profiles), C.c_int, C.POINTER(Profiles),
+                                np.ctypeslib.ndpointer(np.int32, flags="C"), C.c_int32,
+                                C.POINTER(Tables), C.POINTER(Options), C.c_float, C.c_float,
+                                C.c_float, C.c_int]
+        L.or_search.restype = SearchOut
+        L._search_ready = True
+    pros = np.ascontiguousarray(pros, np.int32)
+    return L.or_search(C.byref(q), qp, C.byref(d), pros, pros.size, C.byref(self.t), C.byref(self.o),
+                       scorethld, logevthld, searchspace, nthreads)
+
+
+Oracle.search = _oracle_search

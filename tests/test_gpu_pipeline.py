@@ -62,3 +62,36 @@ def test_pipeline_phase1_finalize_and_stats(rmax):
                 assert abs(got - val) <= 1e-6 * max(1.0, abs(val)), (name, dp, got, val)
         le = res["logE"]
         assert np.all(np.diff(le[res["order"]]) >= 0)
+
+
+def test_score_block_bitexact():
+    q = SynthDB(23, 2, 150, 0.668, 30, 400)
+    d = SynthDB(24, 12, 180, 0.668, 30, 800)
+    orc = oracle()
+    cbp = processing()
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    for qp in range(q.nprof):
+        for dp in range(d.nprof):
+            S, M = cbp.score_block(qp, dp)
+            eS, eM = orc.score_f32(q.view, qp, d.view, dp)
+            assert np.array_equal(S.view(np.uint32), eS.view(np.uint32)), (qp, dp)
+            assert np.array_equal(M.view(np.uint32), eM.view(np.uint32)), (qp, dp)
+
+
+def test_batch_equals_one_by_one():
+    q = SynthDB(25, 4, 150, 0.668, 60, 500)
+    d = SynthDB(26, 120, 180, 0.668, 30, 800)
+    d.plant(q, 27, 20, 3.0, 60.0)
+    cbp = processing()
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    batch = cbp.ProcessBatch([3, 0, 2, 1])
+    for qp, rb in zip([3, 0, 2, 1], batch):
+        r1 = cbp.ProcessScoreMatrix(qp)
+        assert rb["npassed"] == r1["npassed"] and rb["nmap"] == r1["nmap"]
+        assert np.array_equal(rb["alndata"].view(np.uint32), r1["alndata"].view(np.uint32))
+        assert np.array_equal(rb["order"], r1["order"])
+        assert rb["alns"] == r1["alns"]
+    assert sum(r["nmap"] for r in batch) > 0
