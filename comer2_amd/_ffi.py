@@ -34,6 +34,10 @@ class Results(C.Structure):
                 ("search_space", C.c_double)]
 
 
+class SdpPair(C.Structure):
+    _fields_ = [("qidx", C.c_int32), ("pronr", C.c_int32), ("soff", C.c_int64)]
+
+
 NFIELDS = 18
 (F_SCORE, F_ORGPRONO, F_PRONEWDST, F_PSTVS, F_BEGCOORDS, F_ENDCOORDS,
  F_LMBDEST, F_KEST, F_EVALUE, F_EPA, F_MIN, F_MAX, F_E, F_H, F_LMBD, F_K,

@@ -54,6 +54,14 @@ __device__ __forceinline__ float wave_shr1(float v, float old)
         __builtin_bit_cast(int, old), __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
 }
 
+/* whole-wave shift by one lane towards lower lanes (DPP wave_shl:1);
+ * lane 63 receives 'old' */
+__device__ __forceinline__ float wave_shl1(float v, float old)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+        __builtin_bit_cast(int, old), __builtin_bit_cast(int, v), 0x130, 0xf, 0xf, false));
+}
+
 __device__ __forceinline__ void wave_lds_sync()
 {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");

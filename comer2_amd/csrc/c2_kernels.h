@@ -94,6 +94,40 @@ hipError_t c2_launch_stats(const StatParams& sp, const float* nnw /*device [93]*
                            const int32_t* counts, const StatIn* in, int32_t n,
                            double* out /* [n][10] */, hipStream_t stream);
 hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream_t stream);
+
+/* soft DP: one pair of (query, chunk profile) and its workspace offsets (floats) */
+struct SdpPair {
+    int32_t q, p;        /* query index, chunk profile index */
+    int32_t L1, L2;      /* lengths */
+    int32_t ns, ncp;     /* stripes of 64*R rows; checkpoint slots per stripe */
+    int64_t soff;        /* offset of the pair's [L1][L2] block in S / dS / posterior */
+    int64_t foff;        /* forward store: all cells (K = 0) or checkpoints */
+    int64_t hoff;        /* stripe hand-offs: forward [ns][5][L2], then adjoint [3][L2] */
+};
+struct SdpArgs {
+    const SdpPair* pairs; const int32_t* order; int32_t npairs;
+    const float* qtr; int64_t qntr; const uint32_t* qdist;   /* query set transitions [7][npos+nprof] */
+    const float* qll; const float* qld;                      /* column -1 vectors [qnpos][5] */
+    const float* dtr; int64_t dntr; const uint32_t* ddist;   /* chunk transitions */
+    const float* S; const float* theta;                      /* [sum L1*L2], [5] */
+    float* logZ; const float* gZ;                            /* [npairs] */
+    float* dS; float* dth; float* post;                      /* adjoint outputs */
+    float* ws;                                               /* workspace */
+    int64_t scr_off;                                         /* per-wave segment scratch */
+    int32_t K;                                               /* checkpoint interval, 0 = store all */
+};
+int c2_sdp_rows_per_lane();
+int64_t c2_sdp_ckpt_floats(int R);
+hipError_t c2_launch_sdp(const SdpArgs& a, bool adjoint, int nwaves, hipStream_t stream);
+struct SdpScoreArgs {
+    const SdpPair* pairs;
+    const float* qtrg; const float* qss; const float* qbkg; const uint32_t* qdist; int64_t qnpos; int32_t qnprof;
+    const float* dtrg; const float* dss; const float* dbkg; const uint32_t* ddist; int64_t dnpos; int32_t dnprof;
+    const float* ssss;   /* [30][30] */
+    float shift, ssswgt;
+    float* S;
+};
+hipError_t c2_launch_sdp_scores(const SdpScoreArgs& a, int npairs, hipStream_t stream);
 /* S and mod of query vs chunk profile p (Ld its length), row-major [Lq][Ld] */
 hipError_t c2_launch_score_block(const ScoreCtx& c, int p, int Ld, float* S, float* mod, hipStream_t stream);
 

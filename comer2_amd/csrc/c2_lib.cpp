@@ -148,6 +148,8 @@ int cbp_upload_queries(cbp_handle* h, const cbp_profiles* q)
     h->queries.copy_from(q);
     h->have_queries = true;
     h->qcache.clear();
+    h->sdq_ready = false;
+    h->sdp_ready = false;
     return CBP_OK;
 }
 
@@ -191,6 +193,7 @@ int cbp_upload_chunk(cbp_handle* h, const cbp_profiles* db)
     h->chunk.copy_from(db);
     HIPCHK(hipStreamSynchronize(h->stream));
     h->have_chunk = true;
+    h->sdp_ready = false;
     return CBP_OK;
 }
 
@@ -658,5 +661,153 @@ extern "C" int cbp_score_block(cbp_handle* h, int32_t qidx, int32_t pronr, float
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
     h->used -= dS.n + dM.n;
     if (e != hipSuccess) return fail(h, CBP_ERR_HIP, std::string("cbp_score_block: ") + hipGetErrorString(e));
+    return CBP_OK;
+}
+
+
+/* ---- differentiable soft DP ---- */
+
+static int sdp_upload_queries(cbp_handle* h)
+{
+    if (h->sdq_ready) return CBP_OK;
+    const cbp_profiles& q = h->queries.v;
+    int rc;
+    if ((rc = dupload(h, h->sdq_trans, q.trans, (size_t)7 * (q.npos + q.nprof)))) return rc;
+    if ((rc = dupload(h, h->sdq_dist, q.dist, (size_t)q.nprof))) return rc;
+    std::vector<float> ll((size_t)q.npos * 5), ld((size_t)q.npos * 5);
+    for (int32_t p = 0; p < q.nprof; p++)
+        c2_map_colm1(&q, p, ll.data() + (size_t)q.dist[p] * 5, ld.data() + (size_t)q.dist[p] * 5);
+    if ((rc = dupload(h, h->sdq_ll, ll.data(), ll.size()))) return rc;
+    if ((rc = dupload(h, h->sdq_ld, ld.data(), ld.size()))) return rc;
+    if ((rc = dupload(h, h->sdq_trg, q.trgfrq, (size_t)20 * q.npos))) return rc;
+    if ((rc = dupload(h, h->sdq_ss, q.ssprb, (size_t)3 * q.npos))) return rc;
+    if ((rc = dupload(h, h->sdq_bkg, q.bkg, (size_t)20 * q.nprof))) return rc;
+    if ((rc = dupload(h, h->dssss, h->ssss.data(), h->ssss.size()))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->sdq_ready = true;
+    return CBP_OK;
+}
+
+extern "C" int cbp_sdp_prepare(cbp_handle* h, const cbp_sdp_pair* pairs, int32_t npairs, int32_t ckpt,
+                               int64_t* ws_bytes)
+{
+    if (!h || !ws_bytes || npairs < 0 || (npairs > 0 && !pairs)) return CBP_ERR_ARG;
+    if (!h->have_queries || !h->have_chunk)
+        return fail(h, CBP_ERR_STATE, "queries and chunk must be uploaded first");
+    (void)hipSetDevice(h->device);
+    h->sdp_ready = false;
+    int rc;
+    if ((rc = sdp_upload_queries(h))) return rc;
+    const int R = c2_sdp_rows_per_lane();
+    const cbp_profiles& q = h->queries.v;
+    const cbp_profiles& d = h->chunk.v;
+    int maxsteps = 1;
+    for (int32_t i = 0; i < npairs; i++) {
+        if (pairs[i].qidx < 0 || pairs[i].qidx >= q.nprof || pairs[i].pronr < 0 || pairs[i].pronr >= d.nprof ||
+            pairs[i].soff < 0)
+            return fail(h, CBP_ERR_ARG, "soft DP: pair index or offset out of range");
+        maxsteps = std::max(maxsteps, d.len[pairs[i].pronr] + 63);
+    }
+    int K = ckpt;
+    if (K < 0) { K = (int)std::ceil(std::sqrt((double)maxsteps)); if (K < 16) K = 16; }
+    const auto align = [](int64_t x) { return (x + 63) & ~(int64_t)63; };
+    h->sdp_pairs.resize(npairs);
+    int64_t off = 0;
+    for (int32_t i = 0; i < npairs; i++) {
+        SdpPair& P = h->sdp_pairs[i];
+        P.q = pairs[i].qidx; P.p = pairs[i].pronr;
+        P.L1 = q.len[P.q]; P.L2 = d.len[P.p];
+        P.ns = (P.L1 + 64 * R - 1) / (64 * R);
+        const int nsteps = P.L2 + 63;
+        P.ncp = K > 0 ? (nsteps - 1) / K + 1 : 0;
+        P.soff = pairs[i].soff;
+        P.foff = off;
+        off += align(K > 0 ? (int64_t)P.ns * P.ncp * c2_sdp_ckpt_floats(R)
+                           : (int64_t)P.ns * nsteps * 5 * R * 64);
+        P.hoff = off;
+        off += align((int64_t)P.ns * 5 * P.L2 + 3 * (int64_t)P.L2);
+    }
+    /* longest first (cost = stripes x steps), ties by pair index */
+    h->sdp_order.resize(npairs);
+    for (int32_t i = 0; i < npairs; i++) h->sdp_order[i] = i;
+    std::stable_sort(h->sdp_order.begin(), h->sdp_order.end(), [&](int a, int b) {
+        const SdpPair& A = h->sdp_pairs[a]; const SdpPair& B = h->sdp_pairs[b];
+        return (int64_t)A.ns * (A.L2 + 63) > (int64_t)B.ns * (B.L2 + 63);
+    });
+    int nw = std::min(npairs, 2048);
+    nw = ((nw + 3) / 4) * 4;
+    h->sdp_nwaves = nw;
+    h->sdp_scr_off = off;
+    if (K > 0) off += align((int64_t)nw * K * 5 * R * 64);
+    h->sdp_K = K;
+    h->sdp_ws_floats = off;
+    if (npairs > 0) {
+        if ((rc = dupload(h, h->sdp_dpairs, h->sdp_pairs.data(), h->sdp_pairs.size()))) return rc;
+        if ((rc = dupload(h, h->sdp_dorder, h->sdp_order.data(), h->sdp_order.size()))) return rc;
+    }
+    HIPCHK(hipStreamSynchronize(h->stream));
+    *ws_bytes = off * (int64_t)sizeof(float);
+    h->sdp_ready = true;
+    return CBP_OK;
+}
+
+static void sdp_fill(cbp_handle* h, SdpArgs* a, const float* S, const float* theta, void* ws)
+{
+    std::memset(a, 0, sizeof *a);
+    a->pairs = h->sdp_dpairs.as<SdpPair>(); a->order = h->sdp_dorder.as<int32_t>();
+    a->npairs = (int32_t)h->sdp_pairs.size();
+    const cbp_profiles& q = h->queries.v;
+    a->qtr = h->sdq_trans.as<float>(); a->qntr = q.npos + q.nprof; a->qdist = h->sdq_dist.as<uint32_t>();
+    a->qll = h->sdq_ll.as<float>(); a->qld = h->sdq_ld.as<float>();
+    a->dtr = h->dchunk.trans.as<float>(); a->dntr = h->dchunk.npos + h->dchunk.nprof;
+    a->ddist = h->dchunk.dist.as<uint32_t>();
+    a->S = S; a->theta = theta;
+    a->ws = (float*)ws; a->scr_off = h->sdp_scr_off; a->K = h->sdp_K;
+}
+
+extern "C" int cbp_sdp_forward(cbp_handle* h, const float* S, const float* theta, float* logZ,
+                               void* ws, void* stream)
+{
+    if (!h || !S || !theta || !logZ || !ws) return CBP_ERR_ARG;
+    if (!h->sdp_ready) return fail(h, CBP_ERR_STATE, "soft DP: cbp_sdp_prepare first");
+    (void)hipSetDevice(h->device);
+    SdpArgs a;
+    sdp_fill(h, &a, S, theta, ws);
+    a.logZ = logZ;
+    HIPCHK(c2_launch_sdp(a, false, h->sdp_nwaves, stream ? (hipStream_t)stream : h->stream));
+    return CBP_OK;
+}
+
+extern "C" int cbp_sdp_adjoint(cbp_handle* h, const float* S, const float* theta, const float* logZ,
+                               const float* gZ, float* dS, float* dtheta, float* post, void* ws, void* stream)
+{
+    if (!h || !S || !theta || !logZ || !dS || !dtheta || !ws) return CBP_ERR_ARG;
+    if (!h->sdp_ready) return fail(h, CBP_ERR_STATE, "soft DP: cbp_sdp_prepare first");
+    (void)hipSetDevice(h->device);
+    SdpArgs a;
+    sdp_fill(h, &a, S, theta, ws);
+    a.logZ = (float*)logZ; a.gZ = gZ; a.dS = dS; a.dth = dtheta; a.post = post;
+    HIPCHK(c2_launch_sdp(a, true, h->sdp_nwaves, stream ? (hipStream_t)stream : h->stream));
+    return CBP_OK;
+}
+
+extern "C" int cbp_sdp_scores(cbp_handle* h, float* S, void* stream)
+{
+    if (!h || !S) return CBP_ERR_ARG;
+    if (!h->sdp_ready || !h->have_tables || !h->have_opt)
+        return fail(h, CBP_ERR_STATE, "soft DP: tables, options and cbp_sdp_prepare first");
+    (void)hipSetDevice(h->device);
+    const cbp_profiles& q = h->queries.v;
+    SdpScoreArgs a;
+    a.pairs = h->sdp_dpairs.as<SdpPair>();
+    a.qtrg = h->sdq_trg.as<float>(); a.qss = h->sdq_ss.as<float>(); a.qbkg = h->sdq_bkg.as<float>();
+    a.qdist = h->sdq_dist.as<uint32_t>(); a.qnpos = q.npos; a.qnprof = q.nprof;
+    a.dtrg = h->dchunk.trgfrq.as<float>(); a.dss = h->dchunk.ssprb.as<float>(); a.dbkg = h->dchunk.bkg.as<float>();
+    a.ddist = h->dchunk.dist.as<uint32_t>(); a.dnpos = h->dchunk.npos; a.dnprof = h->dchunk.nprof;
+    a.ssss = h->dssss.as<float>();
+    a.shift = h->opt.mapaln ? 0.002f : 0.0003f;
+    a.ssswgt = h->opt.ssswgt;
+    a.S = S;
+    HIPCHK(c2_launch_sdp_scores(a, (int)h->sdp_pairs.size(), stream ? (hipStream_t)stream : h->stream));
     return CBP_OK;
 }

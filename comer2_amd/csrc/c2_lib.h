@@ -87,6 +87,15 @@ struct cbp_handle {
     std::vector<float> res_epa;   /* log pair E-values before MAP reuses the slot */
     std::vector<int> res_map;     /* alndata rows realigned, in alns order */
     std::vector<BatchStore> batch; /* results of the last cbp_process_batch */
+    /* soft DP: query set on the device and the prepared pair plan */
+    DBuf sdq_trans, sdq_dist, sdq_ll, sdq_ld, sdq_trg, sdq_ss, sdq_bkg, dssss;
+    bool sdq_ready = false;
+    std::vector<SdpPair> sdp_pairs;
+    std::vector<int32_t> sdp_order;
+    DBuf sdp_dpairs, sdp_dorder;
+    int32_t sdp_K = 0, sdp_nwaves = 0;
+    int64_t sdp_ws_floats = 0, sdp_scr_off = 0;
+    bool sdp_ready = false;
 };
 
 #endif

@@ -27,6 +27,13 @@ def lib():
         return _lib
     if not os.path.exists(_ffi.LIB_HIP):
         raise CbpError("libcomer2_hip.so is not built (run comer2_amd.build)")
+    # PyTorch ships its own HIP runtime with the same soname (libamdhip64.so.7);
+    # loading it first makes this library bind to it, so a process that also
+    # uses torch GPU tensors has one HIP runtime, not two.
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
     L = C.CDLL(_ffi.LIB_HIP)
     H = C.c_void_p
     L.cbp_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(H)]
@@ -46,6 +53,11 @@ def lib():
     L.cbp_process.argtypes = [H, C.c_int32, C.c_float, C.c_float, C.POINTER(Results)]
     L.cbp_process_batch.argtypes = [H, C.c_void_p, C.c_int32, C.POINTER(Results)]
     L.cbp_score_block.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    V = C.c_void_p
+    L.cbp_sdp_prepare.argtypes = [H, C.POINTER(_ffi.SdpPair), C.c_int32, C.c_int32, C.POINTER(C.c_int64)]
+    L.cbp_sdp_scores.argtypes = [H, V, V]
+    L.cbp_sdp_forward.argtypes = [H, V, V, V, V, V]
+    L.cbp_sdp_adjoint.argtypes = [H, V, V, V, V, V, V, V, V, V]
     _lib = L
     return L
 

@@ -164,6 +164,38 @@ int  cbp_set_tuning(cbp_handle* h, int32_t seg_cols, int32_t rmax, int32_t nw);
  * dominant kernel (ms, HIP events on its stream) and the number of launches */
 int  cbp_last_kernel_ms(const cbp_handle* h, double* ms, int32_t* nlaunch);
 
+/* ---- differentiable soft DP (no reference counterpart) ----
+ * The model is the MAP forward model of ExecMAPDP_Fwd_Unroll32x
+ * (cumapdp/CuBatchMAPDP_fwd.cu:291-372) with exact log-sum-exp, over the
+ * uploaded queries and chunk, plus five global log-offsets theta added to the
+ * transition classes: insert open, insert extend, delete open, delete
+ * extend, gap close.  logZ = log sum_ij exp(MM(i,j)).  The adjoint returns
+ * dlogZ/dS, dlogZ/dtheta and, optionally, the MM posteriors
+ * exp(F+B-logZ), the quantity of CuBatchMAPDP_bwd.cu:531-544.
+ * S, theta, logZ, gZ, dS, dtheta, post and ws are device pointers; stream is
+ * a hipStream_t (NULL: the handle's stream); calls are asynchronous. */
+typedef struct {
+    int32_t qidx;        /* query index in the uploaded query set */
+    int32_t pronr;       /* profile index in the uploaded chunk */
+    int64_t soff;        /* float offset of the pair's row-major [Lq][Ldb] block in S, dS, post */
+} cbp_sdp_pair;
+/* plan the pairs; ckpt = 0 keeps every forward cell for the adjoint, K > 0
+ * keeps a checkpoint every K anti-diagonal steps and recomputes, -1 picks
+ * K = ceil(sqrt(max steps)); returns the workspace size the caller allocates */
+int  cbp_sdp_prepare(cbp_handle* h, const cbp_sdp_pair* pairs, int32_t npairs, int32_t ckpt,
+                     int64_t* ws_bytes);
+/* score matrices of the prepared pairs into S (device), the score of
+ * CalcSM_Init_SMEMUnroll2x + CalcSM_SSSS_SMEMUnroll2x from the raw profile
+ * fields (cusco/CuBatchScoreMatrix_Init.cu:45-186, CuBatchScoreMatrix_ssss.cu:51-280) */
+int  cbp_sdp_scores(cbp_handle* h, float* S, void* stream);
+/* logZ[npairs] (natural log) */
+int  cbp_sdp_forward(cbp_handle* h, const float* S, const float* theta, float* logZ,
+                     void* ws, void* stream);
+/* after cbp_sdp_forward with the same S, theta, ws: gZ [npairs] upstream
+ * gradient (NULL = 1), dS [like S], dtheta [npairs][5], post (NULL or like S) */
+int  cbp_sdp_adjoint(cbp_handle* h, const float* S, const float* theta, const float* logZ,
+                     const float* gZ, float* dS, float* dtheta, float* post, void* ws, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

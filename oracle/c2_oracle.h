@@ -165,3 +165,20 @@ or_search_out or_search(const cbp_profiles* q, int qp, const cbp_profiles* d,
 #ifdef __cplusplus
 }
 #endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+/* --- soft DP (no reference counterpart; model in c2_oracle_sdp.c), FP64 --- */
+/* column -1 boundary vectors of the MAP forward pass, [Lq][5] each */
+void or_map_colm1(const cbp_profiles* q, int qp, float* lleft, float* ldiag);
+/* log Z for score matrix S [Lq][Ld] (natural log units) and theta[5] */
+double or_sdp_logz(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
+                   const double* S, const double* theta, const float* ll, const float* ld);
+/* gradients: dS [Lq][Ld], dtheta[5], post (optional) MM posteriors; returns log Z */
+double or_sdp_grad(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
+                   const double* S, const double* theta, const float* ll, const float* ld,
+                   double* dS, double* dtheta, double* post);
+#ifdef __cplusplus
+}
+#endif

@@ -659,3 +659,15 @@ double or_map_f64(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
     free(Sp); free(ll); free(ldg); free(F); free(prev); free(cur);
     return logZ;
 }
+
+/* column -1 boundary of the MAP forward pass, [Lq][5] each (MM MI IM DN ND) */
+void or_map_colm1(const cbp_profiles* q, int qp, float* lleft, float* ldiag)
+{
+    const int Lq = q->len[qp];
+    st5* a = (st5*)malloc(sizeof(st5) * Lq);
+    st5* b = (st5*)malloc(sizeof(st5) * Lq);
+    map_colm1(q, qp, a, b);
+    for (int r = 0; r < Lq; r++)
+        for (int s = 0; s < 5; s++) { lleft[r * 5 + s] = a[r].v[s]; ldiag[r * 5 + s] = b[r].v[s]; }
+    free(a); free(b);
+}

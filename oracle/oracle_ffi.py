@@ -259,3 +259,50 @@ def _oracle_search(self, q, qp, d, pros, scorethld, logevthld, searchspace, nthr
 
 
 Oracle.search = _oracle_search
+
+
+def _sdp_sigs(L):
+    P = C.POINTER(Profiles)
+    dp = np.ctypeslib.ndpointer(np.float64, flags="C")
+    fp = np.ctypeslib.ndpointer(np.float32, flags="C")
+    L.or_map_colm1.argtypes = [P, C.c_int, fp, fp]
+    L.or_sdp_logz.argtypes = [P, C.c_int, P, C.c_int, dp, dp, fp, fp]
+    L.or_sdp_logz.restype = C.c_double
+    L.or_sdp_grad.argtypes = [P, C.c_int, P, C.c_int, dp, dp, fp, fp, dp, dp, C.c_void_p]
+    L.or_sdp_grad.restype = C.c_double
+    L._sdp_ready = True
+
+
+def _oracle_map_colm1(self, q, qp):
+    if not getattr(self.L, "_sdp_ready", False):
+        _sdp_sigs(self.L)
+    Lq = q.len[qp]
+    a = np.zeros((Lq, 5), np.float32); b = np.zeros((Lq, 5), np.float32)
+    self.L.or_map_colm1(C.byref(q), qp, a, b)
+    return a, b
+
+
+def _oracle_sdp_logz(self, q, qp, d, dp, S, theta, bnd=None):
+    """FP64 soft-DP log Z (natural log) of S [Lq][Ld] and theta[5]."""
+    if not getattr(self.L, "_sdp_ready", False):
+        _sdp_sigs(self.L)
+    ll, ld = bnd if bnd is not None else self.map_colm1(q, qp)
+    return self.L.or_sdp_logz(C.byref(q), qp, C.byref(d), dp, np.ascontiguousarray(S, np.float64),
+                              np.ascontiguousarray(theta, np.float64), ll, ld)
+
+
+def _oracle_sdp_grad(self, q, qp, d, dp, S, theta, bnd=None):
+    """(logZ, dS, dtheta, posterior) of the FP64 soft DP."""
+    if not getattr(self.L, "_sdp_ready", False):
+        _sdp_sigs(self.L)
+    ll, ld = bnd if bnd is not None else self.map_colm1(q, qp)
+    S = np.ascontiguousarray(S, np.float64)
+    dS = np.zeros_like(S); dth = np.zeros(5); post = np.zeros_like(S)
+    z = self.L.or_sdp_grad(C.byref(q), qp, C.byref(d), dp, S, np.ascontiguousarray(theta, np.float64),
+                           ll, ld, dS, dth, post.ctypes.data_as(C.c_void_p))
+    return z, dS, dth, post
+
+
+Oracle.map_colm1 = _oracle_map_colm1
+Oracle.sdp_logz = _oracle_sdp_logz
+Oracle.sdp_grad = _oracle_sdp_grad
