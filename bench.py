@@ -26,6 +26,10 @@ committed under profiles/ when present.
 The CPU baseline runs the oracle's whole-pipeline restatement on a strided
 sample of the same DB (OpenMP over profiles) for a bounded time.
 
+The softdp object reports the differentiable soft DP: forward-only and
+forward+adjoint GCUPS on 4,096 pairs (lengths 50-600) and on 256 pairs
+(lengths 2,000-5,000, sqrt(L) checkpointing vs all cells kept).
+
   python bench.py [--gpus 1] [--steps 10] [--warmup 2] [--cpu-seconds 15]
   python bench.py --copy-calib BYTES     one plain device copy (counter scale)
 """
@@ -95,6 +99,63 @@ def cpu_baseline(q, d, orc, thld, logev, sspace, seconds, nthreads):
                        % (done, d.nprof, stride, cells, npass, nmap, dt))
 
 
+def softdp_bench(reps):
+    """Soft DP (log-sum-exp forward and hand-written adjoint) at the two
+    differentiation workloads: 4,096 pairs with lengths uniform in 50-600
+    (all forward cells kept) and 256 pairs with lengths uniform in 2,000-5,000
+    (sqrt(L) checkpointing, and all cells kept for comparison).  Scores are
+    the pairs' profile-profile score matrices; theta = 0."""
+    import torch
+    from comer2_amd._ffi import SynthDB
+    from comer2_amd.host import CuBatchProcessing
+    from comer2_amd.softdp import SoftDP
+    out = {}
+    for name, seed, n, lo, hi, modes in (("pairs_50_600", 4, 4096, 50, 600, (0,)),
+                                         ("pairs_2000_5000", 5, 256, 2000, 5000, (-1, 0))):
+        q = SynthDB(seed, n, 0, 0, lo, hi, sort_desc=False)
+        d = SynthDB(seed + 1000, n, 0, 0, lo, hi, sort_desc=False)
+        cbp = CuBatchProcessing(0)
+        cbp.TransferQueries(q)
+        cbp.TransferCPMData(d)
+        pairs = [(k, k) for k in range(n)]
+        rec = {"pairs": n, "lengths": [lo, hi], "seeds": [seed, seed + 1000]}
+        for ck in modes:
+            torch.cuda.reset_peak_memory_stats()
+            sdp = SoftDP(cbp, pairs, ckpt=ck)
+            S = sdp.scores()
+            th = torch.zeros(5, dtype=torch.float32, device="cuda")
+            logZ = sdp.forward(S, th)
+            dS, dth, _ = sdp.adjoint(S, th, logZ)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(reps):
+                logZ = sdp.forward(S, th)
+            torch.cuda.synchronize()
+            tf = (time.perf_counter() - t) / reps
+            t = time.perf_counter()
+            for _ in range(reps):
+                logZ = sdp.forward(S, th)
+                dS, dth, _ = sdp.adjoint(S, th, logZ)
+            torch.cuda.synchronize()
+            tb = (time.perf_counter() - t) / reps
+            key = "all_cells" if ck == 0 else "checkpointed"
+            rec[key] = {"cells": sdp.total, "ckpt_steps": "auto" if ck < 0 else 0,
+                        "fwd_ms": round(tf * 1e3, 3), "fwd_adj_ms": round(tb * 1e3, 3),
+                        "fwd_gcups": round(sdp.total / tf / 1e9, 3),
+                        "fwd_adj_gcups": round(sdp.total / tb / 1e9, 3),
+                        "ratio": round(tf / tb, 4),
+                        "workspace_bytes": int(sdp.ws_bytes),
+                        "peak_torch_bytes": int(torch.cuda.max_memory_allocated())}
+            log("bench: soft DP %s %s: fwd %.2f ms, fwd+adj %.2f ms" % (name, key, tf * 1e3, tb * 1e3))
+            del sdp, S, dS, dth, logZ
+            torch.cuda.empty_cache()
+        if "checkpointed" in rec and "all_cells" in rec:
+            rec["recompute_overhead"] = round(rec["checkpointed"]["fwd_adj_ms"] / rec["all_cells"]["fwd_adj_ms"] - 1.0, 4)
+        out[name] = rec
+        cbp.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -104,6 +165,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--nprof", type=int, default=N_PROFILES)
     ap.add_argument("--copy-calib", type=int, default=0)
+    ap.add_argument("--no-softdp", action="store_true")
+    ap.add_argument("--softdp-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the search shards over DB profiles "
@@ -190,7 +253,10 @@ def main():
             "avg_launch_ms": round(kavg, 4),
         },
         "cpu_baseline": None,
+        "softdp": None,
     }
+    if not args.no_softdp:
+        out["softdp"] = softdp_bench(args.softdp_reps)
     if not args.no_cpu:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         from oracle_ffi import Oracle

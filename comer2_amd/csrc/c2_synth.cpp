@@ -12,7 +12,7 @@
  * by 0.4 and zero stored as -32768 (cupro/IOProfileModel.cu:363).
  * Per profile: ENO ~ U(1,14), background = bg, and one extra "beginning"
  * transition row.  Lengths ~ round(LogNormal(log(median), sigma)) clipped to
- * [minlen, maxlen]; the database is sorted by length, longest first, as
+ * [minlen, maxlen] (or uniform on it); the database is sorted by length, longest first, as
  * makedb stores it (libpro/srcpro/Db.h:268-275).
  *
  * Randomness: Philox4x32-10 keyed by (seed, profile serial), so every profile
@@ -127,8 +127,9 @@ struct SynthDB {
 
 extern "C" {
 
-/* lengths: LogNormal(log(median), sigma) rounded, clipped; sorted descending
- * when sort_desc != 0.  Returns an owned handle; free with c2s_free. */
+/* lengths: LogNormal(log(median), sigma) rounded, clipped, or uniform on
+ * [minlen, maxlen] when median <= 0; sorted descending when sort_desc != 0.
+ * Returns an owned handle; free with c2s_free. */
 void* c2s_generate(uint64_t seed, int32_t nprof, double median, double sigma,
                    int32_t minlen, int32_t maxlen, int32_t sort_desc, double conc)
 {
@@ -137,8 +138,9 @@ void* c2s_generate(uint64_t seed, int32_t nprof, double median, double sigma,
     {
         Philox g(seed, 0xFFFFFFFFull);
         for (int32_t p = 0; p < nprof; p++) {
-            double l = std::exp(std::log(median) + sigma * g.normal());
-            long v = std::lround(l);
+            long v;
+            if (median > 0.0) v = std::lround(std::exp(std::log(median) + sigma * g.normal()));
+            else v = minlen + (long)(g.uniform() * (double)(maxlen - minlen + 1));
             if (v < minlen) v = minlen;
             if (v > maxlen) v = maxlen;
             lens[p] = (int32_t)v;
