@@ -26,6 +26,10 @@ committed under profiles/ when present.
 The CPU baseline runs the oracle's whole-pipeline restatement on a strided
 sample of the same DB (OpenMP over profiles) for a bounded time.
 
+The batch object reports 64 queries (lengths 50-1000, homologs planted)
+against the same DB, one by one vs one batched call, with the hit lists
+compared bit for bit.
+
 The softdp object reports the differentiable soft DP: forward-only and
 forward+adjoint GCUPS on 4,096 pairs (lengths 50-600) and on 256 pairs
 (lengths 2,000-5,000, sqrt(L) checkpointing vs all cells kept).
@@ -99,6 +103,55 @@ def cpu_baseline(q, d, orc, thld, logev, sspace, seconds, nthreads):
                        % (done, d.nprof, stride, cells, npass, nmap, dt))
 
 
+def batch_bench(d, args):
+    """64 queries with lengths uniform in 50-1000 against the same DB: one by
+    one (plain per-query pipeline) vs one batched call; hit lists compared."""
+    import numpy as np
+    from comer2_amd._ffi import SynthDB
+    from comer2_amd.host import CuBatchProcessing
+    nq = 64
+    q = SynthDB(3, nq, 0, 0, 50, 1000, sort_desc=False)
+    cbp = CuBatchProcessing(0, 0, 0, options=dict(max_map_pairs=100))
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    qs = list(range(nq))
+    lens = q.lengths()
+    cells = int(lens.sum()) * d.npos
+    cbp.ProcessBatch(qs[:4])                          # warm-up: buffers, query tables
+    t = time.perf_counter()
+    plain = [cbp.ProcessScoreMatrix(i) for i in qs]
+    tp = time.perf_counter() - t
+    t = time.perf_counter()
+    batched = cbp.ProcessBatch(qs)
+    tb = time.perf_counter() - t
+    # phase 1 replayed from a HIP graph (captured on the second call)
+    cbp.set_graph(True)
+    cbp.ProcessBatch(qs)
+    cbp.ProcessBatch(qs)
+    t = time.perf_counter()
+    graphed = cbp.ProcessBatch(qs)
+    tg = time.perf_counter() - t
+    cbp.set_graph(False)
+
+    def same_hits(xs, ys):
+        return all(a["npassed"] == b["npassed"] and a["nmap"] == b["nmap"]
+                   and np.array_equal(a["alndata"].view(np.uint32), b["alndata"].view(np.uint32))
+                   and np.array_equal(a["order"], b["order"]) and np.array_equal(a["aln_raw"], b["aln_raw"])
+                   for a, b in zip(xs, ys))
+    same = same_hits(plain, batched) and same_hits(plain, graphed)
+    out = {"queries": nq, "query_lengths": [50, 1000], "seed": 3, "cells": cells,
+           "plain_s": round(tp, 3), "batched_s": round(tb, 3), "graph_s": round(tg, 3),
+           "plain_gcups": round(cells / tp / 1e9, 3), "batched_gcups": round(cells / tb / 1e9, 3),
+           "graph_gcups": round(cells / tg / 1e9, 3),
+           "ratio": round(tp / tb, 4), "ratio_graph": round(tp / tg, 4), "identical_hits": bool(same),
+           "passed": int(sum(r["npassed"] for r in batched)), "realigned": int(sum(r["nmap"] for r in batched)),
+           "map_cap_per_query": 100}
+    log("bench: 64 queries plain %.2f s, batched %.2f s, graph %.2f s, identical %s" % (tp, tb, tg, same))
+    cbp.close()
+    return out
+
+
 def softdp_bench(reps):
     """Soft DP (log-sum-exp forward and hand-written adjoint) at the two
     differentiation workloads: 4,096 pairs with lengths uniform in 50-600
@@ -166,6 +219,7 @@ def main():
     ap.add_argument("--nprof", type=int, default=N_PROFILES)
     ap.add_argument("--copy-calib", type=int, default=0)
     ap.add_argument("--no-softdp", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--softdp-reps", type=int, default=3)
     args = ap.parse_args()
     if args.gpus != 1:
@@ -184,6 +238,10 @@ def main():
     q = SynthDB(QUERY_SEED, 1, QUERY_LEN, 0.0, QUERY_LEN, QUERY_LEN)
     d.plant(q, PLANT_SEED, N_PLANT if args.nprof == N_PROFILES else max(1, args.nprof // 50),
             PLANT_KMIN, PLANT_KMAX)
+    if not args.no_batch:
+        qb = SynthDB(3, 64, 0, 0, 50, 1000, sort_desc=False)
+        d.plant(qb, PLANT_SEED + 1, 20 * 64 if args.nprof == N_PROFILES else max(64, args.nprof // 100),
+                PLANT_KMIN, PLANT_KMAX)
     log("bench: generated %d profiles, %d positions in %.1f s" % (d.nprof, d.npos, time.perf_counter() - t))
 
     cbp = CuBatchProcessing(0, 0, 0, options=dict(max_map_pairs=MAP_CAP))
@@ -254,7 +312,10 @@ def main():
         },
         "cpu_baseline": None,
         "softdp": None,
+        "batch": None,
     }
+    if not args.no_batch:
+        out["batch"] = batch_bench(d, args)
     if not args.no_softdp:
         out["softdp"] = softdp_bench(args.softdp_reps)
     if not args.no_cpu:

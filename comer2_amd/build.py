@@ -11,7 +11,7 @@ CSRC = os.path.join(HERE, "csrc")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HIP_SOURCES = ["c2_phase1.hip", "c2_phase2.hip", "c2_map.hip", "c2_sdp.hip", "c2_lib.cpp", "c2_prep.cpp"]
+HIP_SOURCES = ["c2_phase1.hip", "c2_phase2.hip", "c2_map.hip", "c2_sdp.hip", "c2_lib.cpp", "c2_pipe.cpp", "c2_prep.cpp"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
              "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]

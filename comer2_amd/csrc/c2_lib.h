@@ -40,13 +40,48 @@ struct DevQuery {
     DBuf qv, qtr, qtrb, QT, qh, lleft, ldiag, qbkg, qaa, mleft, mdiag;
 };
 
-/* per-query results of one cbp_process_batch call */
-struct BatchStore {
+/* device and host state of one query's pass through the stage sequence;
+ * results stay valid until the next cbp_process* call */
+struct QWork {
+    int qidx = -1;
+    DevQuery* dq = nullptr;
+    float scorethld = 0.0f, logevthld = 0.0f;
+    hipStream_t st = nullptr;
+    int slot = 0;                       /* stream slot (stripe hand-off buffer) */
+    DBuf keys;                          /* [nprof] phase-1 maxima */
+    unsigned long long* hkeys = nullptr; /* pinned host copy of keys */
+    size_t hkeys_n = 0;
+    std::vector<int32_t> cand, pe;
+    std::vector<int64_t> off;
+    DBuf dcand, dpe, doff, btck, fin;
+    std::vector<FinRec> hfin;
+    std::vector<int32_t> pass;
+    std::vector<int> passfin;
+    std::vector<StatIn> si;
+    DBuf pass_pro, counts, statin, statout;
+    std::vector<double> hstat;
+    std::vector<int> sel;
+    std::vector<MapPair> mp;
+    int64_t alnstride = 0;
+    DBuf mpairs, mF, mSd, mhand, mbt, mstat, malns, malntmp, mres;
+    std::vector<int32_t> hmres;
+    int64_t nposits = 0;
+    int32_t maxlen = 0;
+    uint64_t sspace = 0;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    /* results */
     std::vector<float> aln, epa;
     std::vector<int32_t> order;
     std::vector<char> strs;
     std::vector<int> map;
+    ~QWork() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        if (hkeys) (void)hipHostFree(hkeys);
+    }
 };
+
+constexpr int C2_NSTREAMS = 4;
 
 struct cbp_handle {
     int device = 0;
@@ -70,7 +105,7 @@ struct cbp_handle {
     DBuf dnnw;           /* SS18 parameters on the device */
     /* per-call scratch */
     DBuf cand, cand_off, btck, fin, pass_pro, counts, statin, statout;
-    DBuf mpairs, mF, mbt, mstat, malns, malntmp, mres;
+    DBuf mpairs, mF, mSd, mhand, mbt, mstat, malns, malntmp, mres;
     int32_t nseg = 0;
     DBuf keys;           /* [nprof] u64 */
     DBuf bot;            /* stripe hand-off */
@@ -80,13 +115,20 @@ struct cbp_handle {
     int nw = 8;          /* tuning: waves per workgroup */
     double last_ms = 0.0;
     int32_t last_nlaunch = 0;
-    cbp_results res{};
-    std::vector<float> res_aln;
-    std::vector<int32_t> res_order;
-    std::vector<char> res_strs;
-    std::vector<float> res_epa;   /* log pair E-values before MAP reuses the slot */
-    std::vector<int> res_map;     /* alndata rows realigned, in alns order */
-    std::vector<BatchStore> batch; /* results of the last cbp_process_batch */
+    /* staged pipeline: per-query work, extra streams (default priority) and
+     * one stripe hand-off buffer per stream */
+    std::vector<std::unique_ptr<QWork>> work;
+    hipStream_t xstream[C2_NSTREAMS - 1] = {nullptr, nullptr, nullptr};
+    DBuf sbot[C2_NSTREAMS];
+    /* phase-1 stage of a batch captured in a HIP graph and replayed while the
+     * batch (queries, chunk, launch geometry) is unchanged; a few batches are
+     * kept, keyed by (generation, geometry, query list) */
+    struct GraphEnt { hipGraphExec_t exec = nullptr; int seen = 0; };
+    bool use_graph = false;
+    std::map<std::vector<int32_t>, GraphEnt> graphs;
+    hipEvent_t gfork = nullptr, gjoin[C2_NSTREAMS - 1] = {nullptr, nullptr, nullptr};
+    hipEvent_t gt0 = nullptr, gt1 = nullptr;
+    uint64_t chunk_gen = 0;             /* bumped by every chunk upload / tuning change */
     /* soft DP: query set on the device and the prepared pair plan */
     DBuf sdq_trans, sdq_dist, sdq_ll, sdq_ld, sdq_trg, sdq_ss, sdq_bkg, dssss;
     bool sdq_ready = false;
@@ -97,5 +139,27 @@ struct cbp_handle {
     int64_t sdp_ws_floats = 0, sdp_scr_off = 0;
     bool sdp_ready = false;
 };
+
+/* internal helpers shared by the library's translation units */
+int c2_fail(cbp_handle* h, int code, const std::string& msg);
+int c2_dalloc(cbp_handle* h, DBuf& b, size_t n);
+template <class T>
+inline int c2_dupload(cbp_handle* h, DBuf& b, const T* src, size_t count, hipStream_t st) {
+    int rc = c2_dalloc(h, b, sizeof(T) * count);
+    if (rc) return rc;
+    if (count) {
+        const hipError_t e = hipMemcpyAsync(b.p, src, sizeof(T) * count, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return c2_fail(h, CBP_ERR_HIP, std::string("upload: ") + hipGetErrorString(e));
+    }
+    return CBP_OK;
+}
+DevQuery* c2_get_query(cbp_handle* h, int qidx, int* rc);
+int c2_fill_p1args(cbp_handle* h, DevQuery* dq, P1Args* a);
+void c2_fill_scorectx(cbp_handle* h, DevQuery* dq, ScoreCtx* c);
+uint64_t c2_length_adjustment(const cbp_options& o, uint64_t m, uint64_t n, uint64_t N, uint64_t* delta);
+int c2_build_segments(cbp_handle* h, const cbp_profiles* s);
+int c2_run_phase1(cbp_handle* h, int qidx);
+int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* scorethld,
+                   const float* logevthld, cbp_results* outs);
 
 #endif

@@ -284,7 +284,21 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
     }
 }
 
+__global__ void k_zero_u64(unsigned long long* p, int64_t n)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
 }  // namespace
+
+/* clears the phase-1 maxima with a kernel (safe inside a captured graph) */
+hipError_t c2_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
 
 size_t c2_phase1_lds_bytes(int R, int NW, int nstripes)
 {
@@ -326,8 +340,13 @@ hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream
     const int grid = (a.nseg + NW - 1) / NW;
     const size_t lds = c2_phase1_lds_bytes(R, NW, a.nstripes);
     if (btck) {
-        if (NW != 1) return hipErrorInvalidValue;
-        return launch_nw<1, true>(a, R, grid, lds, stream);
+        switch (NW) {
+            case 8: return launch_nw<8, true>(a, R, grid, lds, stream);
+            case 4: return launch_nw<4, true>(a, R, grid, lds, stream);
+            case 2: return launch_nw<2, true>(a, R, grid, lds, stream);
+            case 1: return launch_nw<1, true>(a, R, grid, lds, stream);
+            default: return hipErrorInvalidValue;
+        }
     }
     switch (NW) {
         case 8: return launch_nw<8, false>(a, R, grid, lds, stream);

@@ -17,16 +17,19 @@
  *
  * k_stats replaces CalcStatParamsDynPlmProfile and the SS18/Obs significance
  * kernels (cuss/CuBatchSS_dynplm.cu:193-674, cuss/CuBatchSS_{lambda,entropy,K,
- * SS18,NN}.cuh, cusf/*): one thread per pair, FP64 throughout.
+ * SS18,NN}.cuh, cusf/*): one wave per pair, FP64 throughout; the score
+ * distribution lives in LDS, the sums over its 672 entries are spread over
+ * the lanes and combined by a fixed xor butterfly, so the Newton/bisection
+ * decisions are wave-uniform and the result does not depend on scheduling.
  */
 #include "c2_score.h"
 
 namespace {
 
-__device__ __forceinline__ float score_at(const ScoreCtx& c, int y, int64_t col, int p)
+__device__ __forceinline__ float score_at(const ScoreCtx& c, int y, int64_t col, const float* w)
 {
     float ss;
-    return c2_score_cell(c, y, col, p, &ss);
+    return c2_score_cell_w(c, y, col, w, &ss);
 }
 
 __global__ void k_finalize(ScoreCtx c, const int32_t* cand, int32_t ncand,
@@ -45,6 +48,8 @@ __global__ void k_finalize(ScoreCtx c, const int32_t* cand, int32_t ncand,
     const uint32_t* bt = btck + btck_off[i];
     int x = xe, y = ye, psts = 0;
     int b = 3;
+    float pw[20];
+    c2_pair_weights(c, p, pw);
     float cache[36];
     for (int k = 0; k < 36; k++) cache[k] = 0.0f;
     float sumcorr = 0.0f;
@@ -60,7 +65,7 @@ __global__ void k_finalize(ScoreCtx c, const int32_t* cand, int32_t ncand,
                 if (b == 0) break;
                 if (b == 2) { y--; continue; }
                 if (b == 1) { x--; continue; }
-                const float sv = score_at(c, y, col0 + x, p);
+                const float sv = score_at(c, y, col0 + x, pw);
                 cache[ndx] = sv;
                 if (sv > 0.0f) psts++;
                 x--; y--; ndx++;
@@ -153,77 +158,6 @@ __device__ void interval(int y, int* l, int* r) {
     if (u <= 0) { left++; if (left < 0) left = 0; }
     if (0 <= u) { rght--; if (NSC - 1 < rght) rght = NSC - 1; }
     *l = left; *r = rght;
-}
-
-__device__ void lambda_f(double x, const double* P, const int* L, double mn, double mx, double* f, double* df) {
-    double lf = 0.0, ldf = 0.0;
-    for (int y = 0; y < NUNS; y++)
-        for (int t = 0; t < SCL; t++) {
-            double sc = L[y] + t + SSMIN;
-            if (mn <= sc && sc <= mx) {
-                sc -= mx;
-                const double v = x * sc;
-                if (LOG_SP_MIN < v) { const double e = exp(v) * P[y * SCL + t]; lf += e; ldf += e * sc; }
-            }
-        }
-    double z = x * -mx;
-    if (LOG_SP_MIN < z) { z = exp(z); lf -= z; ldf += z * mx; }
-    *f = lf; if (df) *df = ldf;
-}
-
-__device__ double find_lambda(const double* P, const int* L, double mn, double mx) {
-    const double x1 = 1.0e-5, x2 = 10.0, acc = 1.0e-4;
-    double f, df, fl, dx, dxold, xh, xl, rts;
-    lambda_f(x1, P, L, mn, mx, &fl, nullptr);
-    lambda_f(x2, P, L, mn, mx, &f, nullptr);
-    if ((fl > 0.0 && f > 0.0) || (fl < 0.0 && f < 0.0)) return -1.0;
-    if (fl < 0.0) { xl = x1; xh = x2; } else { xh = x1; xl = x2; }
-    rts = 0.5 * (x1 + x2); dxold = fabs(x2 - x1); dx = dxold;
-    lambda_f(rts, P, L, mn, mx, &f, &df);
-    int n;
-    for (n = 0; n < 32; n++) {
-        bool stop = false;
-        if ((((rts - xh) * df - f) * ((rts - xl) * df - f) > 0.0) || (fabs(2.0 * f) > fabs(dxold * df))) {
-            dxold = dx; dx = 0.5 * (xh - xl); rts = xl + dx;
-            if (xl == rts) stop = true;
-        } else {
-            dxold = dx; dx = f / df;
-            const double tmp = rts; rts -= dx;
-            if (tmp == rts) stop = true;
-        }
-        if (fabs(dx) < acc) stop = true;
-        if (stop) break;
-        lambda_f(rts, P, L, mn, mx, &f, &df);
-        if (f < 0.0) xl = rts; else xh = rts;
-    }
-    return n == 32 ? -1.0 : rts;
-}
-
-__device__ double prob_len_j(double* C, double* nw, const double* pu, const int* L, int minu, int maxu,
-                             int range, int minsc, int maxsc, double lambda) {
-    double pj = 0.0;
-    for (int y = 0; y < NUNS; y++)
-        for (int t = 0; t < SCL; t++) {
-            double pji = 0.0;
-            const int sc = L[y] + t + SSMIN;
-            if (minsc <= sc && sc <= maxsc && -32 <= sc && sc <= 32) {
-                const double etols = (sc < 0) ? lambda * (double)sc : 0.0;
-                if (LOG_SP_MIN < etols) {
-                    const int lo = (maxsc - sc < range) ? maxu - (maxsc - sc) : minu;
-                    const int hi = (sc - minsc < range) ? minu + (sc - minsc) : maxu;
-                    for (int i = lo; i <= hi; i++) {
-                        int yy = y, xx = t - i;
-                        if (SCL <= xx) { xx -= SCL; if (yy + 1 < NUNS) yy++; else continue; }
-                        else if (xx < 0) { xx += SCL; if (0 < yy) yy--; else continue; }
-                        pji += C[yy * SCL + xx] * pu[i - SMIN];
-                    }
-                    pj += pji * exp(etols);
-                }
-            }
-            nw[y * SCL + t] = pji;
-        }
-    for (int i = 0; i < NUNS * SCL; i++) C[i] = nw[i];
-    return pj;
 }
 
 __device__ double incbcf(double a, double b, double x) {
@@ -338,48 +272,167 @@ __device__ void nnet(const double in[4], const float* w, double factor, double* 
     *out = (2.0 / (1.0 + exp(-o)) - 1.0) * factor;
 }
 
-__global__ __launch_bounds__(64) void k_stats(StatParams sp, const float* nnw, const int32_t* counts,
-                                              const StatIn* in, int32_t n, double* out)
+/* wave-wide sums / extrema with a fixed xor-butterfly order (every lane gets the result) */
+__device__ __forceinline__ double wsum(double v)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    for (int off = 32; off >= 1; off >>= 1) v += __shfl_xor(v, off);
+    return v;
+}
+__device__ __forceinline__ double wmin(double v)
+{
+    for (int off = 32; off >= 1; off >>= 1) { const double o = __shfl_xor(v, off); v = o < v ? o : v; }
+    return v;
+}
+__device__ __forceinline__ double wmax(double v)
+{
+    for (int off = 32; off >= 1; off >>= 1) { const double o = __shfl_xor(v, off); v = o > v ? o : v; }
+    return v;
+}
+
+/* sum_s p(s) e^{x s} - 1 on the scaled scores (GetFdFlmbdequation,
+ * cuss/CuBatchSS_lambda.cuh:79-170), entries spread over the lanes */
+__device__ void lambda_fw(double x, const double* P, const int* L, double mn, double mx, int lane,
+                          double* f, double* df)
+{
+    double lf = 0.0, ldf = 0.0;
+    for (int j = lane; j < NUNS * SCL; j += 64) {
+        const int y = j / SCL, t = j - y * SCL;
+        double sc = L[y] + t + SSMIN;
+        if (mn <= sc && sc <= mx) {
+            sc -= mx;
+            const double v = x * sc;
+            if (LOG_SP_MIN < v) { const double e = exp(v) * P[j]; lf += e; ldf += e * sc; }
+        }
+    }
+    lf = wsum(lf); ldf = wsum(ldf);
+    double z = x * -mx;
+    if (LOG_SP_MIN < z) { z = exp(z); lf -= z; ldf += z * mx; }
+    *f = lf; if (df) *df = ldf;
+}
+
+/* root of the lambda equation by Newton-Raphson safeguarded with bisection
+ * (FindLmbdRootByNRandBisection, cuss/CuBatchSS_lambda.cuh:199-398) */
+__device__ double find_lambda_w(const double* P, const int* L, double mn, double mx, int lane)
+{
+    const double x1 = 1.0e-5, x2 = 10.0, acc = 1.0e-4;
+    double f, df, fl, dx, dxold, xh, xl, rts;
+    lambda_fw(x1, P, L, mn, mx, lane, &fl, &df);
+    lambda_fw(x2, P, L, mn, mx, lane, &f, &df);
+    if ((fl > 0.0 && f > 0.0) || (fl < 0.0 && f < 0.0)) return -1.0;
+    if (fl < 0.0) { xl = x1; xh = x2; } else { xh = x1; xl = x2; }
+    rts = 0.5 * (x1 + x2); dxold = fabs(x2 - x1); dx = dxold;
+    lambda_fw(rts, P, L, mn, mx, lane, &f, &df);
+    int it;
+    for (it = 0; it < 32; it++) {
+        bool stop = false;
+        if ((((rts - xh) * df - f) * ((rts - xl) * df - f) > 0.0) || (fabs(2.0 * f) > fabs(dxold * df))) {
+            dxold = dx; dx = 0.5 * (xh - xl); rts = xl + dx;
+            if (xl == rts) stop = true;
+        } else {
+            dxold = dx; dx = f / df;
+            const double tmp = rts; rts -= dx;
+            if (tmp == rts) stop = true;
+        }
+        if (fabs(dx) < acc) stop = true;
+        if (stop) break;
+        lambda_fw(rts, P, L, mn, mx, lane, &f, &df);
+        if (f < 0.0) xl = rts; else xh = rts;
+    }
+    return it == 32 ? -1.0 : rts;
+}
+
+/* probability of alignment length j by convolution (CalcProbOfAlnLengthj,
+ * cuss/CuBatchSS_K.cuh:214-321); C (= P) advanced in place through NW */
+__device__ double prob_len_jw(double* C, double* NW, const double* pu, const int* L, int minu, int maxu,
+                              int range, int minsc, int maxsc, double lambda, int lane)
+{
+    double pj = 0.0;
+    for (int j = lane; j < NUNS * SCL; j += 64) {
+        const int y = j / SCL, t = j - y * SCL;
+        double pji = 0.0;
+        const int sc = L[y] + t + SSMIN;
+        if (minsc <= sc && sc <= maxsc && -32 <= sc && sc <= 32) {
+            const double etols = (sc < 0) ? lambda * (double)sc : 0.0;
+            if (LOG_SP_MIN < etols) {
+                const int lo = (maxsc - sc < range) ? maxu - (maxsc - sc) : minu;
+                const int hi = (sc - minsc < range) ? minu + (sc - minsc) : maxu;
+                for (int i = lo; i <= hi; i++) {
+                    int yy = y, xx = t - i;
+                    if (SCL <= xx) { xx -= SCL; if (yy + 1 < NUNS) yy++; else continue; }
+                    else if (xx < 0) { xx += SCL; if (0 < yy) yy--; else continue; }
+                    pji += C[yy * SCL + xx] * pu[i - SMIN];
+                }
+                pj += pji * exp(etols);
+            }
+        }
+        NW[j] = pji;
+    }
+    pj = wsum(pj);
+    wave_lds_sync();
+    for (int j = lane; j < NUNS * SCL; j += 64) C[j] = NW[j];
+    wave_lds_sync();
+    return pj;
+}
+
+/* Karlin-Altschul statistics and E-values of one passed pair per wave
+ * (CalcStatParamsDynPlmProfile + CalcSignificanceSS18/ObsDynPlmProfile,
+ * cuss/CuBatchSS_dynplm.cu:193-674), FP64 */
+__global__ __launch_bounds__(256) void k_stats(StatParams sp, const float* nnw, const int32_t* counts,
+                                               const StatIn* in, int32_t n, double* out)
+{
+    __shared__ double sP[4][NUNS * SCL], sNW[4][NUNS * SCL], spu[4][NUNS];
+    __shared__ int sL[4][NUNS], sR[4][NUNS];
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + wv;
     if (i >= n) return;
-    double P[NUNS * SCL], NW[NUNS * SCL], pu[NUNS];
-    int L[NUNS], Rr[NUNS];
+    double* P = sP[wv]; double* NW = sNW[wv]; double* pu = spu[wv];
+    int* L = sL[wv]; int* Rr = sR[wv];
     const int32_t* cnt = counts + (size_t)i * NSC;
-    double tot = 0.0;
-    for (int y = 0; y < NUNS; y++) {
-        interval(y, &L[y], &Rr[y]);
+    if (lane < NUNS) {
+        int l, r;
+        interval(lane, &l, &r);
+        L[lane] = l; Rr[lane] = r;
         double s = 0.0;
         for (int t = 0; t < SCL; t++) {
-            P[y * SCL + t] = (t + L[y] <= Rr[y]) ? (double)cnt[L[y] + t] : 0.0;
-            s += P[y * SCL + t];
+            P[lane * SCL + t] = (t + l <= r) ? (double)cnt[l + t] : 0.0;
+            s += P[lane * SCL + t];
         }
-        pu[y] = s; tot += s;
+        pu[lane] = s;
     }
-    for (int y = 0; y < NUNS; y++) { pu[y] /= tot; for (int t = 0; t < SCL; t++) P[y * SCL + t] /= tot; }
+    wave_lds_sync();
+    double tot = 0.0;
+    for (int y = 0; y < NUNS; y++) tot += pu[y];
+    wave_lds_sync();
+    for (int j = lane; j < NUNS * SCL; j += 64) P[j] /= tot;
+    if (lane < NUNS) pu[lane] /= tot;
+    wave_lds_sync();
     double E = 0.0, mn = 0.0, mx = 0.0;
-    for (int y = 0; y < NUNS; y++)
-        for (int t = 0; t < SCL; t++) {
-            const double sc = L[y] + t + SSMIN;
-            E += sc * P[y * SCL + t];
-            const double v = P[y * SCL + t] != 0.0 ? sc : 0.0;
-            if (v < mn) mn = v;
-            if (v > mx) mx = v;
-        }
-    double lmb = (0.0 <= E || mx <= 0.0) ? -1.0 : find_lambda(P, L, mn, mx);
+    for (int j = lane; j < NUNS * SCL; j += 64) {
+        const int y = j / SCL, t = j - y * SCL;
+        const double sc = L[y] + t + SSMIN;
+        E += sc * P[j];
+        const double v = P[j] != 0.0 ? sc : 0.0;
+        if (v < mn) mn = v;
+        if (v > mx) mx = v;
+    }
+    E = wsum(E); mn = wmin(mn); mx = wmax(mx);
+    double lmb = (0.0 <= E || mx <= 0.0) ? -1.0 : find_lambda_w(P, L, mn, mx, lane);
     double H = -1.0;
     if (lmb > 0.0) {
         H = 0.0;
-        for (int y = 0; y < NUNS; y++)
-            for (int t = 0; t < SCL; t++) {
-                const double sc = L[y] + t + SSMIN;
-                if (mn <= sc && sc <= mx) {
-                    const double v = lmb * sc;
-                    if (LOG_SP_MIN < v && v < LOG_SP_MAX - 3.0) H += exp(v) * v * P[y * SCL + t];
-                }
+        for (int j = lane; j < NUNS * SCL; j += 64) {
+            const int y = j / SCL, t = j - y * SCL;
+            const double sc = L[y] + t + SSMIN;
+            if (mn <= sc && sc <= mx) {
+                const double v = lmb * sc;
+                if (LOG_SP_MIN < v && v < LOG_SP_MAX - 3.0) H += exp(v) * v * P[j];
             }
+        }
+        H = wsum(H);
     }
-    for (int t = 0; t < SCL; t++) P[(0 - SMIN) * SCL + t] = 1.0;
+    wave_lds_sync();
+    if (lane < SCL) P[(0 - SMIN) * SCL + lane] = 1.0;
+    wave_lds_sync();
     if (lmb > 0.0) lmb *= SCL;
     E /= SCL;
     int minu = 0, maxu = 0;
@@ -399,12 +452,13 @@ __global__ __launch_bounds__(64) void k_stats(StatParams sp, const float* nnw, c
         int minsc = minu, maxsc = maxu;
         const int range = maxu - minu + 1;
         for (int j = 1; j <= 32 && 1.0e-4 < Pj; j++) {
-            Pj = prob_len_j(P, NW, pu, L, minu, maxu, range, minsc, maxsc, lmb);
+            Pj = prob_len_jw(P, NW, pu, L, minu, maxu, range, minsc, maxsc, lmb, lane);
             sigma += Pj / j;
             minsc += minu; maxsc += maxu;
         }
         K = lmb * exp(-2.0 * sigma) / (H * (1.0 - exp(-lmb)));
     }
+    if (lane != 0) return;
     double* o = out + (size_t)i * 10;
     o[0] = minu; o[1] = maxu; o[2] = E; o[3] = H; o[4] = lmb; o[5] = K;
     const double lref = sp.ung_lambda, kref = sp.ung_K, lg = sp.gap_lambda, kg = sp.gap_K;
@@ -512,7 +566,7 @@ hipError_t c2_launch_stats(const StatParams& sp, const float* nnw, const int32_t
                            const StatIn* in, int32_t n, double* out, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stats, dim3((n + 63) / 64), dim3(64), 0, stream, sp, nnw, counts, in, n, out);
+    hipLaunchKernelGGL(k_stats, dim3((n + 3) / 4), dim3(256), 0, stream, sp, nnw, counts, in, n, out);
     return hipGetLastError();
 }
 

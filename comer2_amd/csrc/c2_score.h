@@ -48,4 +48,41 @@ __device__ __forceinline__ float c2_score_cell(const ScoreCtx& c, int y, int64_t
     return sc;
 }
 
+/* the pair's 20 weights 1/(0.5 (bq_a + bd_a)), computed as c2_score_cell does */
+__device__ __forceinline__ void c2_pair_weights(const ScoreCtx& c, int p, float* w)
+{
+#pragma unroll
+    for (int k = 0; k < 20; k++) w[k] = 1.0f / (0.5f * (c.qbkg[k] + c.bkg[(size_t)k * c.nprof + p]));
+}
+
+/* c2_score_cell with the pair's weights precomputed (same FP32 result) */
+__device__ __forceinline__ float c2_score_cell_w(const ScoreCtx& c, int y, int64_t col, const float* w, float* ssout)
+{
+    float acc = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 20; k++) {
+        const float dw = c.trgfrq[(size_t)k * c.npos + col] * w[k];
+        const float qk = c.qv[(size_t)k * c.Lpad + y];
+        acc = (k == 0) ? qk * dw : fmaf(qk, dw, acc);
+    }
+    float sc = c2_logf_clamped(acc) - c.shift, ss = 0.0f;
+    if (c.ssswgt > 0.0f) {
+        float pj[3]; int co[3];
+#pragma unroll
+        for (int j = 0; j < 3; j++) {
+            pj[j] = c.ssprb[(size_t)j * c.npos + col];
+            co[j] = 10 * j + (int)(pj[j] * 9.999f);
+        }
+        const float dh = (pj[2] + 0.1f) * 0.1111111f;
+        ss = c.QT[(size_t)co[0] * c.Lpad + y] * pj[0];
+        ss = fmaf(c.QT[(size_t)co[1] * c.Lpad + y], pj[1], ss);
+        ss = fmaf(c.QT[(size_t)co[2] * c.Lpad + y], pj[2], ss);
+        ss = fmaf(c.qh[y], dh, ss);
+        ss = (ss - 0.05f) * c.ssswgt;
+        sc = sc + ss;
+    }
+    *ssout = ss;
+    return sc;
+}
+
 #endif

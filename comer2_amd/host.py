@@ -50,6 +50,8 @@ def lib():
     L.cbp_phase1_keys.argtypes = [H, C.c_int32, np.ctypeslib.ndpointer(np.uint64, flags="C")]
     L.cbp_last_kernel_ms.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.cbp_set_tuning.argtypes = [H, C.c_int32, C.c_int32, C.c_int32]
+    L.cbp_set_graph.argtypes = [H, C.c_int32]
+    L.cbp_autotune.argtypes = [H, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.cbp_process.argtypes = [H, C.c_int32, C.c_float, C.c_float, C.POINTER(Results)]
     L.cbp_process_batch.argtypes = [H, C.c_void_p, C.c_int32, C.POINTER(Results)]
     L.cbp_score_block.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -172,6 +174,18 @@ class CuBatchProcessing:
 
     def set_tuning(self, seg_cols=4096, rmax=5, nw=8):
         self._check(self.L.cbp_set_tuning(self.h, seg_cols, rmax, nw))
+
+    def set_graph(self, on=True):
+        """Replay the phase-1 stage from a captured HIP graph while the batch,
+        chunk and tuning stay unchanged."""
+        self._check(self.L.cbp_set_graph(self.h, 1 if on else 0))
+
+    def autotune(self, qidx=0):
+        """Times phase 1 of query qidx over a small grid of segment widths and
+        waves per workgroup; keeps and returns the fastest (seg_cols, nw)."""
+        sc, nw = C.c_int32(), C.c_int32()
+        self._check(self.L.cbp_autotune(self.h, qidx, C.byref(sc), C.byref(nw)))
+        return sc.value, nw.value
 
 
 def results_to_dict(r):

@@ -160,6 +160,16 @@ int  cbp_score_block(cbp_handle* h, int32_t qidx, int32_t pronr, float* S, float
  * cudp/CuBatchDP_com.h:16-30) */
 int  cbp_set_tuning(cbp_handle* h, int32_t seg_cols, int32_t rmax, int32_t nw);
 
+/* autotuner: times the phase-1 launch of query qidx over segment widths
+ * {2048, 4096, 8192, 16384} columns and {4, 8} waves per workgroup, keeps the
+ * fastest (cbp_set_tuning) and returns it; no reference counterpart */
+int  cbp_autotune(cbp_handle* h, int32_t qidx, int32_t* seg_cols, int32_t* nw);
+
+/* capture the phase-1 stage of cbp_process / cbp_process_batch in a HIP graph
+ * and replay it while the batch, chunk and tuning are unchanged (the first
+ * run of a batch is eager); no reference counterpart */
+int  cbp_set_graph(cbp_handle* h, int32_t on);
+
 /* device-side elapsed time of the last cbp_phase1_keys / cbp_process call's
  * dominant kernel (ms, HIP events on its stream) and the number of launches */
 int  cbp_last_kernel_ms(const cbp_handle* h, double* ms, int32_t* nlaunch);

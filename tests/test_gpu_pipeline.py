@@ -95,3 +95,50 @@ def test_batch_equals_one_by_one():
         assert np.array_equal(rb["order"], r1["order"])
         assert rb["alns"] == r1["alns"]
     assert sum(r["nmap"] for r in batch) > 0
+
+
+def test_graph_replay_matches_eager():
+    """Phase 1 captured in a HIP graph: every replay, with eager work and
+    synchronisations between replays, gives the eager result bit for bit."""
+    q = SynthDB(28, 3, 150, 0.668, 60, 500)
+    d = SynthDB(29, 100, 180, 0.668, 30, 800)
+    d.plant(q, 30, 16, 3.0, 60.0)
+    cbp = processing()
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    qs = [2, 0, 1]
+    eager = cbp.ProcessBatch(qs)
+    cbp.set_graph(True)
+    for rep in range(4):
+        got = cbp.ProcessBatch(qs)
+        for e, g in zip(eager, got):
+            assert g["npassed"] == e["npassed"] and g["nmap"] == e["nmap"], rep
+            assert np.array_equal(g["alndata"].view(np.uint32), e["alndata"].view(np.uint32)), rep
+            assert np.array_equal(g["order"], e["order"]), rep
+            assert g["alns"] == e["alns"], rep
+        # eager work between replays: a single query and a score block
+        r1 = cbp.ProcessScoreMatrix(1)
+        assert np.array_equal(r1["alndata"].view(np.uint32), eager[2]["alndata"].view(np.uint32))
+        cbp.score_block(0, rep)
+    # a tuning change invalidates the graph; the re-captured one still agrees
+    cbp.set_tuning(2048, 5, 4)
+    for rep in range(2):
+        got = cbp.ProcessBatch(qs)
+        for e, g in zip(eager, got):
+            assert np.array_equal(g["alndata"].view(np.uint32), e["alndata"].view(np.uint32)), rep
+    cbp.set_graph(False)
+
+
+def test_autotune_keeps_results():
+    q = SynthDB(31, 2, 150, 0.668, 60, 500)
+    d = SynthDB(32, 200, 180, 0.668, 30, 800)
+    cbp = processing()
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    before = cbp.phase1_keys(0)
+    sc, nw = cbp.autotune(0)
+    assert sc in (2048, 4096, 8192, 16384) and nw in (4, 8)
+    assert np.array_equal(cbp.phase1_keys(0), before)
+    assert np.array_equal(cbp.phase1_keys(1), cbp.phase1_keys(1))
