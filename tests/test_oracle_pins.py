@@ -249,3 +249,78 @@ def test_soft_dp_reproduces_map_posteriors(orc):
             z, dS, dth, post = orc.sdp_grad(q.view, qp, d.view, dp, Sd.astype(np.float64), np.zeros(5))
             assert abs(z - r.logZ) < 1e-4 * abs(z)
             assert np.max(np.abs(post - np.exp(FB.astype(np.float64) - r.logZ))) < 1e-4
+
+
+def _planted_path(moves, vals, Lq=160, Ld=200, seed=0):
+    """Score matrix and backtrack codes of a hand-made path.  `moves` lists the
+    steps walked back from the end cell ('D' diagonal, 'U' up, 'L' left);
+    vals[k] is the score of the k-th diagonal cell met on that walk."""
+    rng = np.random.default_rng(seed)
+    S = rng.uniform(-3, 3, (Lq, Ld)).astype(np.float32)
+    bt = np.zeros((Lq, Ld), np.int8)                       # OR_STOP everywhere
+    ex, ey = Ld - 5, Lq - 3
+    x, y, k = ex, ey, 0
+    code = {"D": 3, "U": 2, "L": 1}
+    for mv in moves:
+        bt[y, x] = code[mv]
+        if mv == "D":
+            S[y, x] = vals[k]; k += 1; x -= 1; y -= 1
+        elif mv == "U":
+            y -= 1
+        else:
+            x -= 1
+    assert k == len(vals) and x >= 0 and y >= 0
+    m = of.DpMax(); m.maxsc = 10.0; m.x = ex; m.y = ey
+    return S, bt, m, (x + 1, y + 1), (ex, ey)
+
+
+@pytest.mark.parametrize("n", [1, 2, 3, 4, 5, 31, 32, 33, 36, 70, 101])
+def test_finalize_constant_path_closed_form(orc, n):
+    """Constant score c on n diagonal cells (gaps between them skipped):
+    sum_k s_k (s_{k-1} + ... + s_{k-4}) = c^2 sum_k min(k-1, 4), so the bonus
+    is sqrt(c^2 sum_k min(k-1, 4) / 2); windows of 32 cells carry 4 scores over."""
+    c = 0.75
+    moves = []
+    for k in range(n):
+        moves.append("D")
+        if k % 7 == 3:
+            moves.append("U")
+        if k % 11 == 5:
+            moves += ["L", "L"]
+    S, bt, m, beg, end = _planted_path(moves, [c] * n)
+    f = orc.finalize(S, bt, m)
+    tot = sum(min(k - 1, 4) for k in range(1, n + 1))
+    want = 10.0 + math.sqrt(0.5 * c * c * tot)
+    assert abs(f.score - want) <= 2e-6 * want, (n, f.score, want)
+    assert f.psts == n
+    assert (f.begx, f.begy) == beg and (f.endx, f.endy) == end
+
+
+@pytest.mark.parametrize("n", [4, 32, 70])
+def test_finalize_alternating_path_closed_form(orc, n):
+    """Scores c(-1)^k: every k >= 5 contributes -c^2 + c^2 - c^2 + c^2 = 0,
+    k = 2 and k = 4 give -c^2 each, k = 3 gives 0: the sum is -2c^2 and the
+    bonus is -c (the negative branch); positives are the even k."""
+    c = 1.25
+    vals = [c * (-1) ** k for k in range(1, n + 1)]
+    S, bt, m, beg, end = _planted_path(["D"] * n, vals, seed=n)
+    f = orc.finalize(S, bt, m)
+    assert abs(f.score - (10.0 - c)) <= 2e-6 * 10.0
+    assert f.psts == n // 2
+    assert (f.begx, f.begy) == beg
+
+
+def test_finalize_path_to_the_edge(orc):
+    """A path that runs into row 0 ends there: the beginning is (x+1, 0)."""
+    Lq, Ld = 8, 20
+    S = np.full((Lq, Ld), -1.0, np.float32)
+    bt = np.zeros((Lq, Ld), np.int8)
+    for k in range(Lq):
+        bt[Lq - 1 - k, 15 - k] = 3
+        S[Lq - 1 - k, 15 - k] = 2.0
+    m = of.DpMax(); m.maxsc = 5.0; m.x = 15; m.y = Lq - 1
+    f = orc.finalize(S, bt, m)
+    assert (f.begx, f.begy) == (15 - Lq + 1, 0)
+    tot = sum(min(k - 1, 4) for k in range(1, Lq + 1))
+    assert abs(f.score - (5.0 + math.sqrt(0.5 * 4.0 * tot))) <= 1e-5
+    assert f.psts == Lq
