@@ -77,7 +77,13 @@ struct MapArgs {
     int64_t alnstride;
     int32_t* res;                    /* [npairs][8]: alnlen idts psts gaps beg end maxsc logZ */
     float minpp;
+    const int32_t* order;            /* launch order of the pairs (longest first) */
+    int32_t R;                       /* query rows per lane of the MAP passes */
+    int32_t ns;                      /* stripes of 64*R rows */
+    int32_t nw;                      /* waves per workgroup (stripes in flight per pair) */
 };
+/* MAP geometry for a query: rows per lane and waves per pair */
+void c2_map_geometry(int Lq, int* R, int* ns, int* nw);
 hipError_t c2_launch_map(const ScoreCtx& c, const MapArgs& m, int npairs, hipStream_t stream);
 /* floats of a pair's lane-step matrix (stripes x (Ld+63) steps x 64R) and of its hand-off buffers */
 int64_t c2_map_cells(int R, int nstripes, int Ld);
@@ -121,11 +127,12 @@ struct SdpArgs {
     float* dS; float* dth; float* post;                      /* adjoint outputs */
     float* ws;                                               /* workspace */
     int64_t scr_off;                                         /* per-wave segment scratch */
-    int32_t K;                                               /* checkpoint interval, 0 = store all */
+    int32_t K;                                               /* checkpoint interval (divides 64), 0 = store all */
 };
 int c2_sdp_rows_per_lane();
 int64_t c2_sdp_ckpt_floats(int R);
-hipError_t c2_launch_sdp(const SdpArgs& a, bool adjoint, int nwaves, hipStream_t stream);
+/* groups persistent workgroups of nw waves (one stripe each) */
+hipError_t c2_launch_sdp(const SdpArgs& a, bool adjoint, int groups, int nw, hipStream_t stream);
 struct SdpScoreArgs {
     const SdpPair* pairs;
     const float* qtrg; const float* qss; const float* qbkg; const uint32_t* qdist; int64_t qnpos; int32_t qnprof;

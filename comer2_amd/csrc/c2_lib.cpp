@@ -497,8 +497,12 @@ extern "C" int cbp_sdp_prepare(cbp_handle* h, const cbp_sdp_pair* pairs, int32_t
             return fail(h, CBP_ERR_ARG, "soft DP: pair index or offset out of range");
         maxsteps = std::max(maxsteps, d.len[pairs[i].pronr] + 63);
     }
+    /* checkpoint interval: sqrt of the longest step count when automatic,
+     * rounded down to 16, 32 or 64 (the adjoint's phases are 64 steps apart
+     * between stripes of a band, so the interval divides 64) */
     int K = ckpt;
-    if (K < 0) { K = (int)std::ceil(std::sqrt((double)maxsteps)); if (K < 16) K = 16; }
+    if (K < 0) K = (int)std::ceil(std::sqrt((double)maxsteps));
+    if (K > 0) K = K >= 64 ? 64 : (K >= 32 ? 32 : 16);
     const auto align = [](int64_t x) { return (x + 63) & ~(int64_t)63; };
     h->sdp_pairs.resize(npairs);
     int64_t off = 0;
@@ -523,11 +527,15 @@ extern "C" int cbp_sdp_prepare(cbp_handle* h, const cbp_sdp_pair* pairs, int32_t
         const SdpPair& A = h->sdp_pairs[a]; const SdpPair& B = h->sdp_pairs[b];
         return (int64_t)A.ns * (A.L2 + 63) > (int64_t)B.ns * (B.L2 + 63);
     });
-    int nw = std::min(npairs, 2048);
-    nw = ((nw + 3) / 4) * 4;
-    h->sdp_nwaves = nw;
+    /* workgroups of up to 8 waves (one stripe each), about 4096 waves in all */
+    int maxns = 1;
+    for (const SdpPair& P : h->sdp_pairs) maxns = std::max(maxns, P.ns);
+    const int nw = std::min(maxns, 8);
+    const int groups = std::max(1, std::min(npairs, 4096 / nw));
+    h->sdp_nw = nw;
+    h->sdp_groups = groups;
     h->sdp_scr_off = off;
-    if (K > 0) off += align((int64_t)nw * K * 5 * R * 64);
+    if (K > 0) off += align((int64_t)groups * nw * K * 5 * R * 64);
     h->sdp_K = K;
     h->sdp_ws_floats = off;
     if (npairs > 0) {
@@ -563,7 +571,7 @@ extern "C" int cbp_sdp_forward(cbp_handle* h, const float* S, const float* theta
     SdpArgs a;
     sdp_fill(h, &a, S, theta, ws);
     a.logZ = logZ;
-    HIPCHK(c2_launch_sdp(a, false, h->sdp_nwaves, stream ? (hipStream_t)stream : h->stream));
+    HIPCHK(c2_launch_sdp(a, false, h->sdp_groups, h->sdp_nw, stream ? (hipStream_t)stream : h->stream));
     return CBP_OK;
 }
 
@@ -576,7 +584,7 @@ extern "C" int cbp_sdp_adjoint(cbp_handle* h, const float* S, const float* theta
     SdpArgs a;
     sdp_fill(h, &a, S, theta, ws);
     a.logZ = (float*)logZ; a.gZ = gZ; a.dS = dS; a.dth = dtheta; a.post = post;
-    HIPCHK(c2_launch_sdp(a, true, h->sdp_nwaves, stream ? (hipStream_t)stream : h->stream));
+    HIPCHK(c2_launch_sdp(a, true, h->sdp_groups, h->sdp_nw, stream ? (hipStream_t)stream : h->stream));
     return CBP_OK;
 }
 

@@ -62,8 +62,9 @@ struct QWork {
     std::vector<double> hstat;
     std::vector<int> sel;
     std::vector<MapPair> mp;
+    std::vector<int32_t> morder;
     int64_t alnstride = 0;
-    DBuf mpairs, mF, mSd, mhand, mbt, mstat, malns, malntmp, mres;
+    DBuf mpairs, dmorder, mF, mSd, mhand, mbt, mstat, malns, malntmp, mres;
     std::vector<int32_t> hmres;
     int64_t nposits = 0;
     int32_t maxlen = 0;
@@ -135,7 +136,7 @@ struct cbp_handle {
     std::vector<SdpPair> sdp_pairs;
     std::vector<int32_t> sdp_order;
     DBuf sdp_dpairs, sdp_dorder;
-    int32_t sdp_K = 0, sdp_nwaves = 0;
+    int32_t sdp_K = 0, sdp_nw = 1, sdp_groups = 1;
     int64_t sdp_ws_floats = 0, sdp_scr_off = 0;
     bool sdp_ready = false;
 };

@@ -7,12 +7,16 @@
  * CuBatchMAPDP_map_btck.cu:75-386) and FinalizeMAPDP_MAP
  * (CuBatchMAPDP_map_final.cu:63-431).
  *
- * Decomposition: one wave per realigned pair, the lane-rows wavefront of
- * the phase-1 kernel: lane l owns query rows R*l..R*l+R-1 of a stripe of
- * 64*R rows and handles column t-l at step t; the row above (forward and
+ * Decomposition: one workgroup per realigned pair (longest first), one
+ * wave per stripe of 64*R query rows, up to 8 stripes in flight; inside a
+ * wave the lane-rows wavefront of the phase-1 kernel: lane l owns rows
+ * R*l..R*l+R-1 and handles column t-l at step t; the row above (forward and
  * MAP DP) arrives from lane l-1 by DPP wave_shr:1, the row below (backward)
- * from lane l+1 by wave_shl:1; stripes hand their boundary row over through
- * per-pair HBM buffers.  Per-cell matrices (dynamic score S', forward F,
+ * from lane l+1 by wave_shl:1.  The stripes of a band of waves advance in
+ * lockstep 64 steps apart and pass their boundary row through LDS; bands
+ * pass it through per-pair HBM buffers.  R is chosen from the query length
+ * so that one band usually covers the query: the critical path of a pair is
+ * then about Ld + 64*(stripes-1) steps of R cells.  Per-cell matrices (dynamic score S', forward F,
  * F+B, backtrack codes) live in a "lane-step" layout [stripe][step][row in
  * lane][lane], so every pass reads and writes them coalesced.
  *
@@ -37,11 +41,6 @@ __device__ __forceinline__ float lse2(float a, float b) {
     return mlog(mexp(a - m) + mexp(b - m)) + m;
 }
 __device__ __forceinline__ S5 init5() { S5 s; s.mm = 0.0f; s.mi = s.im = s.dn = s.nd = LP0; return s; }
-
-__device__ __forceinline__ int wave_id()
-{
-    return __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-}
 
 /* lane-step index of (stripe s, step t, row k of the lane, lane) */
 template <int R>
@@ -146,7 +145,7 @@ __global__ __launch_bounds__(256) void k_map_score(ScoreCtx c, MapArgs m)
     if (threadIdx.x == 0) c2_pair_weights(c, p, w);
     __syncthreads();
     float* Sd = m.Sd + mp.foff;
-    const int64_t n = (int64_t)c.nstripes * nsteps * R * 64;
+    const int64_t n = (int64_t)m.ns * nsteps * R * 64;
     for (int64_t i = threadIdx.x; i < n; i += blockDim.x) {
         const int lane = (int)(i & 63);
         const int64_t q = i >> 6;
@@ -164,89 +163,126 @@ __global__ __launch_bounds__(256) void k_map_score(ScoreCtx c, MapArgs m)
     }
 }
 
-/* ---- forward: F_MM of every cell, max F ---- */
+/* ---- forward: F_MM of every cell, max F ----
+ * One workgroup per pair, one wave per stripe of a band of NW stripes.  The
+ * waves of a band advance in lockstep (one barrier per step), wave w 64
+ * steps behind wave w-1, so the last row wave w-1 computes in a step is the
+ * row above wave w's first row for its next step; it crosses in a
+ * double-buffered LDS slot.  Bands hand over through the HBM buffer. */
 template <int R>
-__global__ __launch_bounds__(256) void k_map_fwd(ScoreCtx c, MapArgs m, int npairs)
+__global__ __launch_bounds__(512) void k_map_fwd(ScoreCtx c, MapArgs m)
 {
-    const int pi = wave_id();
-    if (pi >= npairs) return;
-    const int lane = threadIdx.x & 63;
+    __shared__ float hx[2][8][5];
+    __shared__ float wmax[8];
+    const int pi = m.order[blockIdx.x];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, NW = blockDim.x >> 6;
     const MapPair mp = m.pairs[pi];
-    const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63, ns = c.nstripes;
+    const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63, ns = m.ns;
     const size_t ntr = (size_t)(c.npos + c.nprof);
     const size_t db = (size_t)c.dist[p] + p;
     const float* Sd = m.Sd + mp.foff;
     float* F = m.F + mp.foff;
-    float* fh = m.hand + mp.hoff;                 /* [ns][5][Ld] bottom rows */
+    float* fh = m.hand + mp.hoff;                 /* [ns][5][Ld] bottom rows (band boundaries) */
     float mymax = 0.0f;
-    for (int s = 0; s < ns; s++) {
+    const int nb = (ns + NW - 1) / NW;
+    for (int b = 0; b < nb; b++) {
+        const int s = b * NW + w;
+        const bool have = s < ns;
+        const int nwb = min(NW, ns - b * NW);
         int r[R]; bool valid[R]; float qt[R][7];
 #pragma unroll
         for (int k = 0; k < R; k++) {
             r[k] = s * 64 * R + R * lane + k;
-            valid[k] = r[k] < Lq;
+            valid[k] = have && r[k] < Lq;
 #pragma unroll
-            for (int j = 0; j < 7; j++) qt[k][j] = m.qtr[(size_t)j * c.Lpad + r[k]];
+            for (int j = 0; j < 7; j++) qt[k][j] = valid[k] ? m.qtr[(size_t)j * c.Lpad + r[k]] : 0.0f;
         }
         S5 H[R];
 #pragma unroll
         for (int k = 0; k < R; k++) H[k] = init5();
-        S5 U = (lane == 0 && s > 0) ? ld_s5(fh + (size_t)(s - 1) * 5 * Ld, Ld, 0) : init5();
-        S5 D = init5();
+        S5 U = init5(), D = init5();
         /* inputs of the next step are loaded one step ahead */
         float sdn[R], dtn[7];
 #pragma unroll
-        for (int k = 0; k < R; k++) sdn[k] = Sd[lsi<R>(s, 0, k, lane, nsteps)];
+        for (int k = 0; k < R; k++) sdn[k] = have ? Sd[lsi<R>(s, 0, k, lane, nsteps)] : LP0;
         fwd_trn(m.trans, ntr, db, -lane, Ld, dtn);
-        for (int t = 0; t < nsteps; t++) {
-            const int cc = t - lane;
-            float sdc[R], dt[7];
+        const int TT = nsteps + 1 + 64 * (nwb - 1);
+        for (int T = 0; T < TT; T++) {
+            const int t = T - 1 - 64 * w;        /* this wave's step; -1 only takes the row above */
+            if (have && t >= 0 && t < nsteps) {
+                const int cc = t - lane;
+                float sdc[R], dt[7];
 #pragma unroll
-            for (int k = 0; k < R; k++) sdc[k] = sdn[k];
+                for (int k = 0; k < R; k++) sdc[k] = sdn[k];
 #pragma unroll
-            for (int j = 0; j < 7; j++) dt[j] = dtn[j];
-            if (t + 1 < nsteps) {
+                for (int j = 0; j < 7; j++) dt[j] = dtn[j];
+                if (t + 1 < nsteps) {
 #pragma unroll
-                for (int k = 0; k < R; k++) sdn[k] = Sd[lsi<R>(s, t + 1, k, lane, nsteps)];
-                fwd_trn(m.trans, ntr, db, cc + 1, Ld, dtn);
-            }
-            if (cc >= 0 && cc < Ld) {
-                S5 prevk = D;
+                    for (int k = 0; k < R; k++) sdn[k] = Sd[lsi<R>(s, t + 1, k, lane, nsteps)];
+                    fwd_trn(m.trans, ntr, db, cc + 1, Ld, dtn);
+                }
+                if (cc >= 0 && cc < Ld) {
+                    S5 prevk = D;
 #pragma unroll
-                for (int k = 0; k < R; k++) {
-                    S5 lf = H[k], dg = prevk;
-                    if (cc == 0) {
-                        const int rr = valid[k] ? r[k] : 0;
-                        lf.mm = m.mleft[0 * (size_t)Lq + rr]; lf.mi = m.mleft[1 * (size_t)Lq + rr];
-                        lf.im = m.mleft[2 * (size_t)Lq + rr]; lf.dn = m.mleft[3 * (size_t)Lq + rr];
-                        lf.nd = m.mleft[4 * (size_t)Lq + rr];
-                        dg.mm = m.mdiag[0 * (size_t)Lq + rr]; dg.mi = m.mdiag[1 * (size_t)Lq + rr];
-                        dg.im = m.mdiag[2 * (size_t)Lq + rr]; dg.dn = m.mdiag[3 * (size_t)Lq + rr];
-                        dg.nd = m.mdiag[4 * (size_t)Lq + rr];
+                    for (int k = 0; k < R; k++) {
+                        S5 lf = H[k], dg = prevk;
+                        if (cc == 0) {
+                            const int rr = valid[k] ? r[k] : 0;
+                            lf.mm = m.mleft[0 * (size_t)Lq + rr]; lf.mi = m.mleft[1 * (size_t)Lq + rr];
+                            lf.im = m.mleft[2 * (size_t)Lq + rr]; lf.dn = m.mleft[3 * (size_t)Lq + rr];
+                            lf.nd = m.mleft[4 * (size_t)Lq + rr];
+                            dg.mm = m.mdiag[0 * (size_t)Lq + rr]; dg.mi = m.mdiag[1 * (size_t)Lq + rr];
+                            dg.im = m.mdiag[2 * (size_t)Lq + rr]; dg.dn = m.mdiag[3 * (size_t)Lq + rr];
+                            dg.nd = m.mdiag[4 * (size_t)Lq + rr];
+                        }
+                        const S5 up = (k == 0) ? U : H[k - 1];
+                        prevk = H[k];
+                        const int64_t ix = lsi<R>(s, t, k, lane, nsteps);
+                        const S5 n = fcell(dg, up, lf, qt[k], dt, sdc[k]);
+                        H[k] = n;
+                        if (valid[k]) { F[ix] = n.mm; mymax = fmx(mymax, n.mm); }
                     }
-                    const S5 up = (k == 0) ? U : H[k - 1];
-                    prevk = H[k];
-                    const int64_t ix = lsi<R>(s, t, k, lane, nsteps);
-                    const S5 n = fcell(dg, up, lf, qt[k], dt, sdc[k]);
-                    H[k] = n;
-                    if (valid[k]) { F[ix] = n.mm; mymax = fmx(mymax, n.mm); }
-                }
-                if (lane == 63 && s + 1 < ns) {
-                    float* h = fh + (size_t)s * 5 * Ld;
-                    h[0 * Ld + cc] = H[R - 1].mm; h[1 * Ld + cc] = H[R - 1].mi; h[2 * Ld + cc] = H[R - 1].im;
-                    h[3 * Ld + cc] = H[R - 1].dn; h[4 * Ld + cc] = H[R - 1].nd;
+                    if (lane == 63) {
+                        if (w + 1 < nwb) {
+                            float* o = hx[T & 1][w];
+                            o[0] = H[R - 1].mm; o[1] = H[R - 1].mi; o[2] = H[R - 1].im;
+                            o[3] = H[R - 1].dn; o[4] = H[R - 1].nd;
+                        } else if (s + 1 < ns) {
+                            float* h = fh + (size_t)s * 5 * Ld;
+                            h[0 * Ld + cc] = H[R - 1].mm; h[1 * Ld + cc] = H[R - 1].mi; h[2 * Ld + cc] = H[R - 1].im;
+                            h[3 * Ld + cc] = H[R - 1].dn; h[4 * Ld + cc] = H[R - 1].nd;
+                        }
+                    }
                 }
             }
-            const S5 tp = (s > 0 && t + 1 < Ld) ? ld_s5(fh + (size_t)(s - 1) * 5 * Ld, Ld, t + 1) : init5();
-            D = U;
-            U.mm = wave_shr1(H[R - 1].mm, tp.mm); U.mi = wave_shr1(H[R - 1].mi, tp.mi);
-            U.im = wave_shr1(H[R - 1].im, tp.im); U.dn = wave_shr1(H[R - 1].dn, tp.dn);
-            U.nd = wave_shr1(H[R - 1].nd, tp.nd);
+            __syncthreads();
+            if (have && t >= -1 && t + 1 < nsteps) {
+                /* the row above the lane's first row at column t+1 - lane */
+                S5 tp = init5();
+                if (lane == 0 && t + 1 < Ld) {
+                    if (w > 0) {
+                        const float* o = hx[T & 1][w - 1];
+                        tp.mm = o[0]; tp.mi = o[1]; tp.im = o[2]; tp.dn = o[3]; tp.nd = o[4];
+                    } else if (s > 0) {
+                        tp = ld_s5(fh + (size_t)(s - 1) * 5 * Ld, Ld, t + 1);
+                    }
+                }
+                D = U;
+                U.mm = wave_shr1(H[R - 1].mm, tp.mm); U.mi = wave_shr1(H[R - 1].mi, tp.mi);
+                U.im = wave_shr1(H[R - 1].im, tp.im); U.dn = wave_shr1(H[R - 1].dn, tp.dn);
+                U.nd = wave_shr1(H[R - 1].nd, tp.nd);
+            }
         }
-        __threadfence();
+        __syncthreads();
     }
     for (int off = 32; off >= 1; off >>= 1) mymax = fmx(mymax, __shfl_xor(mymax, off));
-    if (lane == 0) m.stat[(size_t)pi * 4 + 0] = mymax;
+    if (lane == 0) wmax[w] = mymax;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float mx = wmax[0];
+        for (int k = 1; k < NW; k++) mx = fmx(mx, wmax[k]);
+        m.stat[(size_t)pi * 4 + 0] = mx;
+    }
 }
 
 /* total forward probability in the reference's block order
@@ -315,28 +351,35 @@ __global__ __launch_bounds__(256) void k_map_total(ScoreCtx c, MapArgs m)
     if (threadIdx.x == 0) m.stat[blockIdx.x * 4 + 1] = mlog(total) + maxF;   /* logZ */
 }
 
-/* ---- backward: F := F + B_MM (in place) ---- */
+/* ---- backward: F := F + B_MM (in place) ----
+ * Bands run bottom-up; inside a band wave w runs 64 steps behind wave w+1
+ * (reverse sweep), which hands its first row (B_MM, B_MI, B_DN) up through a
+ * double-buffered LDS slot; bands hand over through the HBM buffer. */
 template <int R>
-__global__ __launch_bounds__(256) void k_map_bwd(ScoreCtx c, MapArgs m, int npairs)
+__global__ __launch_bounds__(512) void k_map_bwd(ScoreCtx c, MapArgs m)
 {
-    const int pi = wave_id();
-    if (pi >= npairs) return;
-    const int lane = threadIdx.x & 63;
+    __shared__ float hb[2][8][3];
+    const int pi = m.order[blockIdx.x];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, NW = blockDim.x >> 6;
     const MapPair mp = m.pairs[pi];
-    const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63, ns = c.nstripes;
+    const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63, ns = m.ns;
     const size_t ntr = (size_t)(c.npos + c.nprof);
     const size_t db = (size_t)c.dist[p] + p;
     const float* Sd = m.Sd + mp.foff;
     float* F = m.F + mp.foff;
-    float* bh = m.hand + mp.hoff + (size_t)ns * 5 * Ld;   /* [3][Ld]: B_MM, B_MI, B_DN of a stripe's first row */
-    for (int s = ns - 1; s >= 0; s--) {
+    float* bh = m.hand + mp.hoff + (size_t)ns * 5 * Ld;   /* [3][Ld]: B_MM, B_MI, B_DN of a band's first row */
+    const int nb = (ns + NW - 1) / NW;
+    for (int b = nb - 1; b >= 0; b--) {
+        const int s = b * NW + w;
+        const bool have = s < ns;
+        const int nwb = min(NW, ns - b * NW);
         int r[R]; bool valid[R]; float qt[R][7];
 #pragma unroll
         for (int k = 0; k < R; k++) {
             r[k] = s * 64 * R + R * lane + k;
-            valid[k] = r[k] < Lq;
+            valid[k] = have && r[k] < Lq;
 #pragma unroll
-            for (int j = 0; j < 7; j++) qt[k][j] = m.qtrb[(size_t)j * c.Lpad + r[k]];
+            for (int j = 0; j < 7; j++) qt[k][j] = valid[k] ? m.qtrb[(size_t)j * c.Lpad + r[k]] : 0.0f;
         }
         const bool below = (s + 1 < ns);
         /* B and S' of the lane's rows at column c+1 (the previous reverse step);
@@ -350,71 +393,86 @@ __global__ __launch_bounds__(256) void k_map_bwd(ScoreCtx c, MapArgs m, int npai
          * B_MM, S' at column c+1 (sp*) */
         float shmm = 0.0f, shmi = LP0, shdn = LP0, shs = LP0;
         float spmm = 0.0f, sps = LP0;
-        if (lane == 63 && below) {
-            shmm = bh[0 * Ld + Ld - 1]; shmi = bh[1 * Ld + Ld - 1]; shdn = bh[2 * Ld + Ld - 1];
-            shs = Sd[lsi<R>(s + 1, Ld - 1, 0, 0, nsteps)];
-        }
         float sdn[R], fn[R], dxn[7];
 #pragma unroll
         for (int k = 0; k < R; k++) {
             const int64_t ix = lsi<R>(s, nsteps - 1, k, lane, nsteps);
-            sdn[k] = Sd[ix]; fn[k] = F[ix];
+            sdn[k] = have ? Sd[ix] : LP0; fn[k] = have ? F[ix] : 0.0f;
         }
         bwd_trn(m.trans, ntr, db, nsteps - 1 - lane, Ld, dxn);
-        for (int t = nsteps - 1; t >= 0; t--) {
-            const int cc = t - lane;
-            float sdc[R], fc[R], dx[7];
+        const int TT = nsteps + 1 + 64 * (nwb - 1);
+        for (int T = 0; T < TT; T++) {
+            /* this wave's step; nsteps only takes the row below */
+            const int t = nsteps - T + 64 * (nwb - 1 - w);
+            if (have && t >= 0 && t < nsteps) {
+                const int cc = t - lane;
+                float sdc[R], fc[R], dx[7];
 #pragma unroll
-            for (int k = 0; k < R; k++) { sdc[k] = sdn[k]; fc[k] = fn[k]; }
+                for (int k = 0; k < R; k++) { sdc[k] = sdn[k]; fc[k] = fn[k]; }
 #pragma unroll
-            for (int j = 0; j < 7; j++) dx[j] = dxn[j];
-            if (t > 0) {
+                for (int j = 0; j < 7; j++) dx[j] = dxn[j];
+                if (t > 0) {
 #pragma unroll
-                for (int k = 0; k < R; k++) {
-                    const int64_t ix = lsi<R>(s, t - 1, k, lane, nsteps);
-                    sdn[k] = Sd[ix]; fn[k] = F[ix];
-                }
-                bwd_trn(m.trans, ntr, db, cc - 1, Ld, dxn);
-            }
-            if (cc >= 0 && cc < Ld) {
-                float nmi = shmi, ndn = shdn;     /* new B of row k+1 at column c */
-                float omm = spmm, os = sps;       /* old B_MM, S' of row k+1 at column c+1 */
-#pragma unroll
-                for (int k = R - 1; k >= 0; k--) {
-                    const float dgmm = omm, dsc = os;
-                    const S5 rt = cur[k];
-                    omm = cur[k].mm; os = scur[k];
-                    const int64_t ix = lsi<R>(s, t, k, lane, nsteps);
-                    const float sk = sdc[k];
-                    S5 n = init5();
-                    if (valid[k]) {
-                        n = bcell(dgmm, nmi, ndn, rt, qt[k], dx, dsc);
-                        F[ix] = fc[k] + n.mm;
+                    for (int k = 0; k < R; k++) {
+                        const int64_t ix = lsi<R>(s, t - 1, k, lane, nsteps);
+                        sdn[k] = Sd[ix]; fn[k] = F[ix];
                     }
-                    cur[k] = n; scur[k] = sk;
-                    nmi = n.mi; ndn = n.dn;
+                    bwd_trn(m.trans, ntr, db, cc - 1, Ld, dxn);
                 }
-                if (lane == 0 && s > 0) {
-                    bh[0 * Ld + cc] = cur[0].mm; bh[1 * Ld + cc] = cur[0].mi; bh[2 * Ld + cc] = cur[0].dn;
-                }
-            } else {
+                if (cc >= 0 && cc < Ld) {
+                    float nmi = shmi, ndn = shdn;     /* new B of row k+1 at column c */
+                    float omm = spmm, os = sps;       /* old B_MM, S' of row k+1 at column c+1 */
 #pragma unroll
-                for (int k = 0; k < R; k++) { cur[k] = init5(); scur[k] = LP0; }
+                    for (int k = R - 1; k >= 0; k--) {
+                        const float dgmm = omm, dsc = os;
+                        const S5 rt = cur[k];
+                        omm = cur[k].mm; os = scur[k];
+                        const int64_t ix = lsi<R>(s, t, k, lane, nsteps);
+                        const float sk = sdc[k];
+                        S5 n = init5();
+                        if (valid[k]) {
+                            n = bcell(dgmm, nmi, ndn, rt, qt[k], dx, dsc);
+                            F[ix] = fc[k] + n.mm;
+                        }
+                        cur[k] = n; scur[k] = sk;
+                        nmi = n.mi; ndn = n.dn;
+                    }
+                    if (lane == 0) {
+                        if (w > 0) {
+                            float* o = hb[T & 1][w];
+                            o[0] = cur[0].mm; o[1] = cur[0].mi; o[2] = cur[0].dn;
+                        } else if (s > 0) {
+                            bh[0 * Ld + cc] = cur[0].mm; bh[1 * Ld + cc] = cur[0].mi; bh[2 * Ld + cc] = cur[0].dn;
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int k = 0; k < R; k++) { cur[k] = init5(); scur[k] = LP0; }
+                }
             }
-            /* hand the first row to lane l-1; lane 63 takes the stripe below at column t-1-63 */
-            float obmm = 0.0f, obmi = LP0, obdn = LP0, obs = LP0;
-            const int cb = t - 1 - 63;
-            if (lane == 63 && below && cb >= 0 && cb < Ld) {
-                obmm = bh[0 * Ld + cb]; obmi = bh[1 * Ld + cb]; obdn = bh[2 * Ld + cb];
-                obs = Sd[lsi<R>(s + 1, cb, 0, 0, nsteps)];
+            __syncthreads();
+            if (have && t >= 1 && t <= nsteps) {
+                /* the row below for step t-1: lane l+1's first row, or for lane 63
+                 * the stripe below at column t-1-63 */
+                float obmm = 0.0f, obmi = LP0, obdn = LP0, obs = LP0;
+                const int cb = t - 1 - 63;
+                if (lane == 63 && below && cb >= 0 && cb < Ld) {
+                    if (w + 1 < nwb) {
+                        const float* o = hb[T & 1][w + 1];
+                        obmm = o[0]; obmi = o[1]; obdn = o[2];
+                    } else {
+                        obmm = bh[0 * Ld + cb]; obmi = bh[1 * Ld + cb]; obdn = bh[2 * Ld + cb];
+                    }
+                    obs = Sd[lsi<R>(s + 1, cb, 0, 0, nsteps)];
+                }
+                spmm = shmm; sps = shs;
+                shmm = wave_shl1(cur[0].mm, obmm);
+                shmi = wave_shl1(cur[0].mi, obmi);
+                shdn = wave_shl1(cur[0].dn, obdn);
+                shs = wave_shl1(scur[0], obs);
             }
-            spmm = shmm; sps = shs;
-            shmm = wave_shl1(cur[0].mm, obmm);
-            shmi = wave_shl1(cur[0].mi, obmi);
-            shdn = wave_shl1(cur[0].dn, obdn);
-            shs = wave_shl1(scur[0], obs);
         }
-        __threadfence();
+        __syncthreads();
     }
 }
 
@@ -427,18 +485,21 @@ __device__ __forceinline__ bool better(float v, int x, int y, float bv, int bx, 
     return v > bv || (v == bv && (x < bx || (x == bx && y < by)));
 }
 
-/* ---- MAP DP over posteriors (ExecMAPDP_MAP_Btck_Unroll32x) ---- */
+/* ---- MAP DP over posteriors (ExecMAPDP_MAP_Btck_Unroll32x) ----
+ * The band-lockstep schedule of k_map_fwd; the row hand-off is one value. */
 template <int R>
-__global__ __launch_bounds__(256) void k_map_dp(ScoreCtx c, MapArgs m, int npairs)
+__global__ __launch_bounds__(512) void k_map_dp(ScoreCtx c, MapArgs m)
 {
-    const int pi = wave_id();
-    if (pi >= npairs) return;
-    const int lane = threadIdx.x & 63;
+    __shared__ float hv[2][8];
+    __shared__ float wbest[8];
+    __shared__ int wbx[8], wby[8];
+    const int pi = m.order[blockIdx.x];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, NW = blockDim.x >> 6;
     const MapPair mp = m.pairs[pi];
-    const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63, ns = c.nstripes;
+    const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63, ns = m.ns;
     const float* FB = m.F + mp.foff;
     char* bt = m.btck + mp.boff;
-    float* vh = m.hand + mp.hoff + (size_t)ns * 5 * Ld + 3 * (size_t)Ld;   /* [Ld] a stripe's last row */
+    float* vh = m.hand + mp.hoff + (size_t)ns * 5 * Ld + 3 * (size_t)Ld;   /* [Ld] a band's last row */
     const float logZ = m.stat[(size_t)pi * 4 + 1];
     const float minpp = m.minpp, gapreg = m.minpp * 0.5f;
     int e = (int)rintf(0.34f * (float)(Lq > Ld ? Lq : Ld));
@@ -448,63 +509,85 @@ __global__ __launch_bounds__(256) void k_map_dp(ScoreCtx c, MapArgs m, int npair
      * (CuBatchMAPDP_map_btck.cu:231-242): zero on rows r%32 == 31-(Ld-1)%32 */
     const int tl = (Ld - 1) % 32;
     float best = 0.0f; int bx = 0, by = 0;
-    for (int s = 0; s < ns; s++) {
+    const int nb = (ns + NW - 1) / NW;
+    for (int b = 0; b < nb; b++) {
+        const int s = b * NW + w;
+        const bool have = s < ns;
+        const int nwb = min(NW, ns - b * NW);
         int r[R];
 #pragma unroll
         for (int k = 0; k < R; k++) r[k] = s * 64 * R + R * lane + k;
         float V[R];
 #pragma unroll
         for (int k = 0; k < R; k++) V[k] = 0.0f;
-        float U = (lane == 0 && s > 0) ? vh[0] : 0.0f, D = 0.0f;
+        float U = 0.0f, D = 0.0f;
         float fbn[R];
 #pragma unroll
-        for (int k = 0; k < R; k++) fbn[k] = FB[lsi<R>(s, 0, k, lane, nsteps)];
-        for (int t = 0; t < nsteps; t++) {
-            const int cc = t - lane;
-            float fbc[R];
+        for (int k = 0; k < R; k++) fbn[k] = have ? FB[lsi<R>(s, 0, k, lane, nsteps)] : 0.0f;
+        const int TT = nsteps + 1 + 64 * (nwb - 1);
+        for (int T = 0; T < TT; T++) {
+            const int t = T - 1 - 64 * w;
+            if (have && t >= 0 && t < nsteps) {
+                const int cc = t - lane;
+                float fbc[R];
 #pragma unroll
-            for (int k = 0; k < R; k++) fbc[k] = fbn[k];
-            if (t + 1 < nsteps) {
+                for (int k = 0; k < R; k++) fbc[k] = fbn[k];
+                if (t + 1 < nsteps) {
 #pragma unroll
-                for (int k = 0; k < R; k++) fbn[k] = FB[lsi<R>(s, t + 1, k, lane, nsteps)];
-            }
-            if (cc >= 0 && cc < Ld) {
-                float prevk = D;
-#pragma unroll
-                for (int k = 0; k < R; k++) {
-                    const int y = r[k];
-                    const int64_t ix = lsi<R>(s, t, k, lane, nsteps);
-                    const bool ok = in_valid(Lq, Ld, e, y, cc);
-                    const float fb = ok ? fbc[k] : 0.0f;
-                    const float dgv = (y > 0 && cc > 0) ? prevk : 0.0f;
-                    const bool upzero = (cc == Ld - 1 && tl != 31 && (y % 32) == 31 - tl);
-                    const float upv = (y > 0 && !upzero) ? ((k == 0) ? U : V[k - 1]) : 0.0f;
-                    const float lfv = (cc > 0) ? V[k] : 0.0f;
-                    prevk = V[k];
-                    char b = 0;
-                    float v = dgv + mexp(fb - logZ) - minpp;
-                    if (v > 0.0f) b = 3; else v = 0.0f;
-                    if (ok && v > 0.0f && better(v, cc, y, best, bx, by)) { best = v; bx = cc; by = y; }
-                    const float u = upv - gapreg; if (v < u) { v = u; b = 2; }
-                    const float l = lfv - gapreg; if (v < l) { v = l; b = 1; }
-                    V[k] = v;
-                    if (y < Lq) bt[ix] = b;
+                    for (int k = 0; k < R; k++) fbn[k] = FB[lsi<R>(s, t + 1, k, lane, nsteps)];
                 }
-                if (lane == 63 && s + 1 < ns) vh[cc] = V[R - 1];
+                if (cc >= 0 && cc < Ld) {
+                    float prevk = D;
+#pragma unroll
+                    for (int k = 0; k < R; k++) {
+                        const int y = r[k];
+                        const int64_t ix = lsi<R>(s, t, k, lane, nsteps);
+                        const bool ok = in_valid(Lq, Ld, e, y, cc);
+                        const float fb = ok ? fbc[k] : 0.0f;
+                        const float dgv = (y > 0 && cc > 0) ? prevk : 0.0f;
+                        const bool upzero = (cc == Ld - 1 && tl != 31 && (y % 32) == 31 - tl);
+                        const float upv = (y > 0 && !upzero) ? ((k == 0) ? U : V[k - 1]) : 0.0f;
+                        const float lfv = (cc > 0) ? V[k] : 0.0f;
+                        prevk = V[k];
+                        char bc = 0;
+                        float v = dgv + mexp(fb - logZ) - minpp;
+                        if (v > 0.0f) bc = 3; else v = 0.0f;
+                        if (ok && v > 0.0f && better(v, cc, y, best, bx, by)) { best = v; bx = cc; by = y; }
+                        const float u = upv - gapreg; if (v < u) { v = u; bc = 2; }
+                        const float l = lfv - gapreg; if (v < l) { v = l; bc = 1; }
+                        V[k] = v;
+                        if (y < Lq) bt[ix] = bc;
+                    }
+                    if (lane == 63) {
+                        if (w + 1 < nwb) hv[T & 1][w] = V[R - 1];
+                        else if (s + 1 < ns) vh[cc] = V[R - 1];
+                    }
+                }
             }
-            const float tp = (s > 0 && t + 1 < Ld) ? vh[t + 1] : 0.0f;
-            D = U;
-            U = wave_shr1(V[R - 1], tp);
+            __syncthreads();
+            if (have && t >= -1 && t + 1 < nsteps) {
+                float tp = 0.0f;
+                if (lane == 0 && t + 1 < Ld) {
+                    if (w > 0) tp = hv[T & 1][w - 1];
+                    else if (s > 0) tp = vh[t + 1];
+                }
+                D = U;
+                U = wave_shr1(V[R - 1], tp);
+            }
         }
-        __threadfence();
+        __syncthreads();
     }
-    /* canonical maximum over the lanes: larger value, then smaller x, then smaller y */
+    /* canonical maximum: larger value, then smaller x, then smaller y */
     for (int off = 32; off >= 1; off >>= 1) {
         const float ov = __shfl_xor(best, off);
         const int ox = __shfl_xor(bx, off), oy = __shfl_xor(by, off);
         if (ov > 0.0f && better(ov, ox, oy, best, bx, by)) { best = ov; bx = ox; by = oy; }
     }
-    if (lane == 0) {
+    if (lane == 0) { wbest[w] = best; wbx[w] = bx; wby[w] = by; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int k = 1; k < NW; k++)
+            if (wbest[k] > 0.0f && better(wbest[k], wbx[k], wby[k], best, bx, by)) { best = wbest[k]; bx = wbx[k]; by = wby[k]; }
         int32_t* o = m.res + (size_t)pi * 8;
         o[5] = (int32_t)(best > 0.0f ? (((uint32_t)bx << 16) | (uint32_t)by) : 0u);
         o[6] = __float_as_int(best);
@@ -561,12 +644,12 @@ __global__ __launch_bounds__(64) void k_map_final(ScoreCtx c, MapArgs m, int npa
 template <int R>
 hipError_t launch_r(const ScoreCtx& c, const MapArgs& m, int npairs, hipStream_t stream)
 {
-    const int wblocks = (npairs + 3) / 4;
+    const int wg = 64 * m.nw;
     hipLaunchKernelGGL((k_map_score<R>), dim3(npairs), dim3(256), 0, stream, c, m);
-    hipLaunchKernelGGL((k_map_fwd<R>), dim3(wblocks), dim3(256), 0, stream, c, m, npairs);
+    hipLaunchKernelGGL((k_map_fwd<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
     hipLaunchKernelGGL((k_map_total<R>), dim3(npairs), dim3(256), 0, stream, c, m);
-    hipLaunchKernelGGL((k_map_bwd<R>), dim3(wblocks), dim3(256), 0, stream, c, m, npairs);
-    hipLaunchKernelGGL((k_map_dp<R>), dim3(wblocks), dim3(256), 0, stream, c, m, npairs);
+    hipLaunchKernelGGL((k_map_bwd<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
+    hipLaunchKernelGGL((k_map_dp<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
     hipLaunchKernelGGL((k_map_final<R>), dim3((npairs + 63) / 64), dim3(64), 0, stream, c, m, npairs);
     return hipGetLastError();
 }
@@ -576,10 +659,22 @@ hipError_t launch_r(const ScoreCtx& c, const MapArgs& m, int npairs, hipStream_t
 int64_t c2_map_cells(int R, int nstripes, int Ld) { return (int64_t)nstripes * (Ld + 63) * R * 64; }
 int64_t c2_map_hand(int nstripes, int Ld) { return (int64_t)nstripes * 5 * Ld + 4 * (int64_t)Ld; }
 
+/* rows per lane so that one band of at most 8 waves covers the query when
+ * it can (R <= 5); waves = stripes of the first band */
+void c2_map_geometry(int Lq, int* R, int* ns, int* nw)
+{
+    int r = (Lq + 511) / 512;
+    if (r < 1) r = 1;
+    if (r > 5) r = 5;
+    const int n = (Lq + 64 * r - 1) / (64 * r);
+    *R = r; *ns = n; *nw = n < 8 ? n : 8;
+}
+
 hipError_t c2_launch_map(const ScoreCtx& c, const MapArgs& m, int npairs, hipStream_t stream)
 {
     if (npairs <= 0) return hipSuccess;
-    switch (c.R) {
+    if (m.nw < 1 || m.nw > 8 || m.ns < 1 || !m.order) return hipErrorInvalidValue;
+    switch (m.R) {
         case 1: return launch_r<1>(c, m, npairs, stream);
         case 2: return launch_r<2>(c, m, npairs, stream);
         case 3: return launch_r<3>(c, m, npairs, stream);

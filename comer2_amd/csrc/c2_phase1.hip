@@ -29,6 +29,7 @@
  * BTCK=true additionally stores 2-bit backtrack codes packed per (step, lane)
  * for the pairs that pass to the finalize stage.
  */
+#include <type_traits>
 #include "c2_dev.h"
 #include "c2_kernels.h"
 
@@ -166,7 +167,55 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
         const bool wbot = tops && (s + 1 < a.nstripes);
         uint32_t* btw = nullptr;
         if (BTCK) btw = a.btck + a.btck_off[seg] + (size_t)s * nsteps * 64;
+        const bool ssterm = a.ssswgt > 0.0f;
 
+        /* the R cells of one step; FIX: some lane is at a profile's column 0,
+         * where row kf takes a zero diagonal (the reference's 32-row block
+         * boundary) */
+        auto cells = [&](auto fixc, int t, bool first, const float* dw, const float* dt, const float* pj,
+                         int xloc) {
+            constexpr bool FIX = decltype(fixc)::value;
+            int co[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++) co[j] = (10 * j + (int)(pj[j] * 9.999f)) * SL + R * lane;
+            const float dh = (pj[2] + 0.1f) * 0.1111111f;
+            const int u0 = (xloc - y0) - cA - 1;
+            uint32_t bw = 0;
+            S5 saved = D;
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                /* score: log(sum_a q_a d_a w_a) - shift + SS term */
+                float acc = q[k][0] * dw[0];
+#pragma unroll
+                for (int j = 1; j < 20; j++) acc = fmaf(q[k][j], dw[j], acc);
+                float sc = c2_logf_clamped(acc) - a.shift;
+                if (ssterm) {
+                    float ss = sQT[co[0] + k] * pj[0];
+                    ss = fmaf(sQT[co[1] + k], pj[1], ss);
+                    ss = fmaf(sQT[co[2] + k], pj[2], ss);
+                    ss = fmaf(qh[k], dh, ss);
+                    ss = (ss - 0.05f) * a.ssswgt;
+                    sc = sc + ss;
+                }
+                if ((uint32_t)(u0 - k) >= (uint32_t)cW) sc = -9.0f;   /* distant corners */
+                S5 dg = saved;
+                if (FIX && k > 0) {
+                    const bool z = first && kf == k;
+                    dg.mm = z ? 0.0f : dg.mm; dg.mi = z ? 0.0f : dg.mi; dg.im = z ? 0.0f : dg.im;
+                    dg.dn = z ? 0.0f : dg.dn; dg.nd = z ? 0.0f : dg.nd;
+                }
+                const S5 up = (k == 0) ? U : H[k - 1];
+                saved = H[k];
+                const S5 n = dp_cell(dg, up, H[k], qt[k], dt, sc);
+                H[k] = n;
+                if (n.mm > best) { best = n.mm; bpos = (t << 3) | k; }
+                if (BTCK) bw |= dp_btck(n) << (2 * k);
+            }
+            if (BTCK) btw[(size_t)t * 64 + lane] = bw;
+        };
+
+        /* one step t: ring refill, profile starts, cells, stripe hand-off,
+         * row above for step t+1 */
         for (int t = 0; t < nsteps; t++) {
             if (((t + 1) % BLK) == 0) {
                 load_block<R>(a, ring, lane, t + 1, c_begin, c_end, tops, s > 0);
@@ -184,7 +233,6 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
 
             /* profile start: flush, reset to the column -1 boundary */
             const bool first = (xloc == 0);
-            uint32_t fixbits = 0;
             if (__builtin_amdgcn_read_exec() & __ballot(first)) {
                 if (first) {
                     if (curpid >= 0 && best > 0.0f) {
@@ -211,48 +259,8 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
                     D.mm = sLD[0 * SL + sl0]; D.mi = sLD[1 * SL + sl0]; D.im = sLD[2 * SL + sl0];
                     D.dn = sLD[3 * SL + sl0]; D.nd = sLD[4 * SL + sl0];
                 }
-#pragma unroll
-                for (int k = 1; k < R; k++)
-                    if (__ballot(first && kf == k)) fixbits |= 1u << k;
             }
-
-            /* per-column SS table offsets: QT[10j + int(9.999 p'_j)] */
-            int co[3];
-#pragma unroll
-            for (int j = 0; j < 3; j++) co[j] = (10 * j + (int)(pj[j] * 9.999f)) * SL + R * lane;
-            const float dh = (pj[2] + 0.1f) * 0.1111111f;
-            const int u0 = (xloc - y0) - cA - 1;
-
-            S5 saved = D;
-            uint32_t bw = 0;
-#pragma unroll
-            for (int k = 0; k < R; k++) {
-                /* score: log(sum_a q_a d_a w_a) - shift + SS term */
-                float acc = q[k][0] * dw[0];
-#pragma unroll
-                for (int j = 1; j < 20; j++) acc = fmaf(q[k][j], dw[j], acc);
-                float sc = c2_logf_clamped(acc) - a.shift;
-                if (a.ssswgt > 0.0f) {
-                    float ss = sQT[co[0] + k] * pj[0];
-                    ss = fmaf(sQT[co[1] + k], pj[1], ss);
-                    ss = fmaf(sQT[co[2] + k], pj[2], ss);
-                    ss = fmaf(qh[k], dh, ss);
-                    ss = (ss - 0.05f) * a.ssswgt;
-                    sc = sc + ss;
-                }
-                if ((uint32_t)(u0 - k) >= (uint32_t)cW) sc = -9.0f;   /* distant corners */
-                S5 dg = saved;
-                if (k > 0 && (fixbits & (1u << k))) {
-                    if (first && kf == k) dg = s5_zero();
-                }
-                const S5 up = (k == 0) ? U : H[k - 1];
-                saved = H[k];
-                const S5 n = dp_cell(dg, up, H[k], qt[k], dt, sc);
-                H[k] = n;
-                if (n.mm > best) { best = n.mm; bpos = (t << 3) | k; }
-                if (BTCK) bw |= dp_btck(n) << (2 * k);
-            }
-            if (BTCK) btw[(size_t)t * 64 + lane] = bw;
+            cells(std::true_type{}, t, first, dw, dt, pj, xloc);
             if (wbot && lane == 63) {
                 const int64_t c = c_begin + t - 63;
                 if (c >= c_begin && c < c_end) {

@@ -18,15 +18,18 @@
  * Arithmetic is base 2 inside (inputs scaled by log2 e once), with the
  * hardware exp2/log2.
  *
- * Decomposition: persistent waves take pairs with a uniform stride over the
- * longest-first order; a wave sweeps its pair in stripes of 64*R query rows,
- * lane l owning rows R*l..R*l+R-1 and processing column t-l at step t, the
- * row above arriving from lane l-1 by DPP wave_shr:1 (forward) and the row
- * below from lane l+1 by wave_shl:1 (adjoint).  Stripe boundaries go
- * through per-pair HBM buffers.  The forward either stores all five states
- * of every cell (K = 0) or only the wave state every K steps; the adjoint
- * then recomputes each K-step segment from its checkpoint into a per-wave
- * scratch and sweeps it backwards (sqrt(L) checkpointing with K ~ sqrt(L2)).
+ * Decomposition: persistent workgroups take pairs with a uniform stride
+ * over the longest-first order; a pair is cut into stripes of 64*R query
+ * rows, one wave per stripe, lane l owning rows R*l..R*l+R-1 and processing
+ * column t-l at step t, the row above arriving from lane l-1 by DPP
+ * wave_shr:1 (forward) and the row below from lane l+1 by wave_shl:1
+ * (adjoint).  The waves of a workgroup take a band of up to 8 stripes that
+ * advance in lockstep 64 steps apart (one barrier per step), passing their
+ * boundary row through LDS; bands pass it through per-pair HBM buffers.
+ * The forward either stores all five states of every cell (K = 0) or only
+ * the wave state every K steps (K divides 64); the adjoint then recomputes
+ * each K-step segment from its checkpoint into per-wave scratch and sweeps
+ * it backwards (sqrt(L) checkpointing).
  */
 #include "c2_dev.h"
 #include "c2_kernels.h"
@@ -43,6 +46,17 @@ __device__ __forceinline__ float lse2(float a, float b)
 {
     const float m = fmaxf(a, b);
     return m + lg2(1.0f + ex2(fminf(a, b) - m));
+}
+
+/* log-sum-exp of two terms as (max, e^{a-max}, e^{b-max}) with one exp2 */
+__device__ __forceinline__ float lse_parts(float a, float b, float& fa, float& fb)
+{
+    const float m = fmaxf(a, b);
+    const float e = ex2(fminf(a, b) - m);
+    const bool ab = a >= b;
+    fa = ab ? 1.0f : e;
+    fb = ab ? e : 1.0f;
+    return m;
 }
 
 struct Th { float oi, ei, od, ed, cl; };
@@ -107,18 +121,6 @@ struct Stripe {
     float qt[R][7];
 };
 
-/* the wave's index in the grid and the grid's wave count, as wave-uniform
- * (scalar) values; pairs are dealt to waves with a uniform stride over the
- * longest-first order */
-__device__ __forceinline__ int wave_id()
-{
-    return __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6));
-}
-__device__ __forceinline__ int wave_count()
-{
-    return __builtin_amdgcn_readfirstlane((int)((gridDim.x * blockDim.x) >> 6));
-}
-
 template <int R>
 __device__ __forceinline__ void load_stripe(const SdpArgs& a, const SdpPair& P, int s, int lane, Stripe<R>& st)
 {
@@ -145,14 +147,15 @@ __device__ __forceinline__ S5 top_at(const SdpArgs& a, const SdpPair& P, const f
     return v;
 }
 
-/* one forward step t of stripe s; returns with st.H holding the new states
- * (only for active lanes), Z accumulated, bottom hand-off written and U/D
- * advanced for step t+1 */
+/* the cells of forward step t of stripe s (active lanes only), Z
+ * accumulated; lane 63 writes its last row to the HBM hand-off (fhw, when a
+ * stripe follows) and to an LDS slot (lds, when the next stripe runs in the
+ * same band) */
 template <int R>
-__device__ __forceinline__ void fwd_step(const SdpArgs& a, const SdpPair& P, const Stripe<R>& sp,
-                                         FwdState<R>& w, int s, int ns, int t, int lane, const Th& th,
-                                         const float* S, const float* fh, float* fhw,
-                                         float& zm, float& zs, bool accz)
+__device__ __forceinline__ void fwd_cells(const SdpArgs& a, const SdpPair& P, const Stripe<R>& sp,
+                                          FwdState<R>& w, int s, int ns, int t, int lane, const Th& th,
+                                          const float* S, float* fhw, float* lds,
+                                          float& zm, float& zs, bool accz)
 {
     const int c = t - lane;
     const bool act = (c >= 0) && (c < P.L2);
@@ -180,20 +183,41 @@ __device__ __forceinline__ void fwd_step(const SdpArgs& a, const SdpPair& P, con
                 else zs += ex2(f - zm);
             }
         }
-        if (lane == 63 && s + 1 < ns && fhw) {
-            float* h = fhw + (size_t)s * 5 * P.L2;
-            h[0 * P.L2 + c] = w.H[R - 1].mm; h[1 * P.L2 + c] = w.H[R - 1].mi;
-            h[2 * P.L2 + c] = w.H[R - 1].im; h[3 * P.L2 + c] = w.H[R - 1].dn;
-            h[4 * P.L2 + c] = w.H[R - 1].nd;
+        if (lane == 63) {
+            const S5& e = w.H[R - 1];
+            if (s + 1 < ns && fhw) {
+                float* h = fhw + (size_t)s * 5 * P.L2;
+                h[0 * P.L2 + c] = e.mm; h[1 * P.L2 + c] = e.mi; h[2 * P.L2 + c] = e.im;
+                h[3 * P.L2 + c] = e.dn; h[4 * P.L2 + c] = e.nd;
+            }
+            if (lds) { lds[0] = e.mm; lds[1] = e.mi; lds[2] = e.im; lds[3] = e.dn; lds[4] = e.nd; }
         }
     }
-    const S5 tp = top_at(a, P, fh, s, t + 1);
+}
+
+/* the row above for the next step: lane l takes lane l-1's last row, lane 0
+ * the stripe's top boundary tp */
+template <int R>
+__device__ __forceinline__ void fwd_shift(FwdState<R>& w, const S5& tp)
+{
     w.D = w.U;
     w.U.mm = wave_shr1(w.H[R - 1].mm, tp.mm);
     w.U.mi = wave_shr1(w.H[R - 1].mi, tp.mi);
     w.U.im = wave_shr1(w.H[R - 1].im, tp.im);
     w.U.dn = wave_shr1(w.H[R - 1].dn, tp.dn);
     w.U.nd = wave_shr1(w.H[R - 1].nd, tp.nd);
+}
+
+/* one whole forward step with the top boundary from the HBM hand-off (the
+ * adjoint's recomputation, after the forward pass wrote every stripe's) */
+template <int R>
+__device__ __forceinline__ void fwd_step(const SdpArgs& a, const SdpPair& P, const Stripe<R>& sp,
+                                         FwdState<R>& w, int s, int ns, int t, int lane, const Th& th,
+                                         const float* S, const float* fh)
+{
+    float zm = 0.0f, zs = 0.0f;
+    fwd_cells<R>(a, P, sp, w, s, ns, t, lane, th, S, nullptr, nullptr, zm, zs, false);
+    fwd_shift<R>(w, top_at(a, P, fh, s, t + 1));
 }
 
 template <int R>
@@ -250,13 +274,20 @@ __device__ __forceinline__ void get_ckpt(const float* base, int lane, FwdState<R
 template <int R>
 __device__ __forceinline__ int64_t ckpt_floats() { return (int64_t)(5 * R + 10) * 64; }
 
+/* forward: one workgroup per pair (persistent over the longest-first
+ * order), one wave per stripe of a band of NW stripes advancing in lockstep
+ * 64 steps apart; wave w-1's last row crosses to wave w in a double-buffered
+ * LDS slot, every stripe's last row also goes to HBM (band boundaries and
+ * the adjoint's recomputation read it there) */
 template <int R>
-__global__ __launch_bounds__(256) void k_sdp_fwd(SdpArgs a)
+__global__ __launch_bounds__(512) void k_sdp_fwd(SdpArgs a)
 {
-    const int lane = threadIdx.x & 63;
+    __shared__ float hx[2][8][5];
+    __shared__ float wz[8][2];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, NW = blockDim.x >> 6;
     const Th th = load_theta(a.theta);
-    const int nwv = wave_count();
-    for (int slot = wave_id(); slot < a.npairs; slot += nwv) {
+    const float lp0 = -99.0f * L2E;
+    for (int slot = blockIdx.x; slot < a.npairs; slot += gridDim.x) {
         const int pi = a.order[slot];
         const SdpPair P = a.pairs[pi];
         const int ns = P.ns, nsteps = P.L2 + 63;
@@ -264,50 +295,98 @@ __global__ __launch_bounds__(256) void k_sdp_fwd(SdpArgs a)
         float* store = a.ws + P.foff;
         float* fh = a.ws + P.hoff;
         float zm = NEG, zs = 0.0f;
-        for (int s = 0; s < ns; s++) {
+        const int nb = (ns + NW - 1) / NW;
+        for (int b = 0; b < nb; b++) {
+            const int s = b * NW + wv;
+            const bool have = s < ns;
+            const int nwb = min(NW, ns - b * NW);
             Stripe<R> sp;
             load_stripe<R>(a, P, s, lane, sp);
             FwdState<R> w;
-            fwd_init<R>(a, P, fh, s, lane, w);
-            for (int t = 0; t < nsteps; t++) {
-                if (a.K > 0 && t > 0 && (t % a.K) == 0)
-                    put_ckpt<R>(store + ((int64_t)s * P.ncp + t / a.K) * ckpt_floats<R>(), lane, w);
-                fwd_step<R>(a, P, sp, w, s, ns, t, lane, th, S, fh, fh, zm, zs, true);
-                if (a.K == 0 && (t - lane) >= 0 && (t - lane) < P.L2)
-                    put_cells<R>(store + ((int64_t)s * nsteps + t) * 5 * R * 64, lane, w.H);
+#pragma unroll
+            for (int k = 0; k < R; k++) w.H[k] = s5_fill(NEG, NEG);
+            w.U = s5_fill(NEG, NEG); w.D = s5_fill(NEG, NEG);
+            const int TT = nsteps + 1 + 64 * (nwb - 1);
+            for (int T = 0; T < TT; T++) {
+                const int t = T - 1 - 64 * wv;       /* this wave's step; -1 only takes the row above */
+                if (have && t >= 0 && t < nsteps) {
+                    if (a.K > 0 && t > 0 && (t % a.K) == 0)
+                        put_ckpt<R>(store + ((int64_t)s * P.ncp + t / a.K) * ckpt_floats<R>(), lane, w);
+                    fwd_cells<R>(a, P, sp, w, s, ns, t, lane, th, S, fh,
+                                 (wv + 1 < nwb) ? hx[T & 1][wv] : nullptr, zm, zs, true);
+                    if (a.K == 0 && (t - lane) >= 0 && (t - lane) < P.L2)
+                        put_cells<R>(store + ((int64_t)s * nsteps + t) * 5 * R * 64, lane, w.H);
+                }
+                __syncthreads();
+                if (have && t >= -1 && t + 1 < nsteps) {
+                    S5 tp = s5_fill(0.0f, lp0);
+                    const int c = t + 1;
+                    if (lane == 0 && s > 0 && c < P.L2) {
+                        if (wv > 0) {
+                            const float* o = hx[T & 1][wv - 1];
+                            tp.mm = o[0]; tp.mi = o[1]; tp.im = o[2]; tp.dn = o[3]; tp.nd = o[4];
+                        } else {
+                            tp = top_at(a, P, fh, s, c);
+                        }
+                    }
+                    fwd_shift<R>(w, tp);
+                }
             }
-            __threadfence();
+            __syncthreads();
         }
-        /* log Z: deterministic xor-butterfly of the lanes' (max, scaled sum) */
+        /* log Z: deterministic xor-butterfly of the lanes' (max, scaled sum),
+         * then the waves in order */
         for (int off = 32; off >= 1; off >>= 1) {
             const float om = __shfl_xor(zm, off), os = __shfl_xor(zs, off);
             const float m = fmaxf(zm, om);
             zs = zs * ex2(zm - m) + os * ex2(om - m);
             zm = m;
         }
-        if (lane == 0) a.logZ[pi] = (zm + lg2(zs)) * LN2;
+        if (lane == 0) { wz[wv][0] = zm; wz[wv][1] = zs; }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            float m = wz[0][0], t = wz[0][1];
+            for (int k = 1; k < NW; k++) {
+                const float om = wz[k][0], os = wz[k][1];
+                const float mm = fmaxf(m, om);
+                t = t * ex2(m - mm) + os * ex2(om - mm);
+                m = mm;
+            }
+            a.logZ[pi] = (m + lg2(t)) * LN2;
+        }
+        __syncthreads();
     }
 }
 
 /* outside-recursion values a lane hands to the row above: B_MM + S, B_MI, B_DN */
 struct Up3 { float bs, mi, dn; };
 
+/* adjoint: the forward's workgroup and band layout, bands bottom-up; inside
+ * a band wave w runs 64 steps behind wave w+1 in the reverse sweep, which
+ * hands its first row (B_MM+S, B_MI, B_DN) up through a double-buffered LDS
+ * slot.  The sweep goes in phases of KK steps (KK = K, a divisor of 64, or
+ * 64 when every cell is stored): with checkpoints, each wave first
+ * recomputes its phase's forward states from the checkpoint into its own
+ * scratch, then the band takes KK reverse steps in lockstep. */
 template <int R>
-__global__ __launch_bounds__(256) void k_sdp_adj(SdpArgs a)
+__global__ __launch_bounds__(512) void k_sdp_adj(SdpArgs a)
 {
-    const int lane = threadIdx.x & 63;
-    const int gw = wave_id();
-    const int nwv = wave_count();
+    __shared__ float hb[2][8][3];
+    __shared__ double wacc[8][5];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, NW = blockDim.x >> 6;
+    const int gw = blockIdx.x * NW + wv;
     const Th th = load_theta(a.theta);
     float* scr = a.ws + a.scr_off + (int64_t)gw * a.K * 5 * R * 64;
-    for (int slot = gw; slot < a.npairs; slot += nwv) {
+    const int KK = a.K > 0 ? a.K : 64;
+    const int lag = 64 / KK;
+    for (int slot = blockIdx.x; slot < a.npairs; slot += gridDim.x) {
         const int pi = a.order[slot];
         const SdpPair P = a.pairs[pi];
         const int ns = P.ns, nsteps = P.L2 + 63, L2 = P.L2;
         const float* S = a.S + P.soff;
         const float* store = a.ws + P.foff;
         float* fh = a.ws + P.hoff;
-        float* bh = fh + (size_t)ns * 5 * L2;      /* [3][L2]: B_MM+S, B_MI, B_DN of the stripe below's first row */
+        float* bh = fh + (size_t)ns * 5 * L2;      /* [3][L2]: B_MM+S, B_MI, B_DN of a band's first row */
         float* dS = a.dS + P.soff;
         float* post = a.post ? a.post + P.soff : nullptr;
         const float Z = a.logZ[pi] * L2E;
@@ -315,7 +394,12 @@ __global__ __launch_bounds__(256) void k_sdp_adj(SdpArgs a)
         const int64_t db = (int64_t)a.ddist[P.p] + P.p;
         const int64_t qb = (int64_t)a.qdist[P.q] + P.q;
         double acc0 = 0.0, acc1 = 0.0, acc2 = 0.0, acc3 = 0.0, acc4 = 0.0;
-        for (int s = ns - 1; s >= 0; s--) {
+        const int nb = (ns + NW - 1) / NW;
+        const int nsegp = nsteps / KK + 1;         /* phases over steps 0..nsteps */
+        for (int b = nb - 1; b >= 0; b--) {
+            const int s = b * NW + wv;
+            const bool have = s < ns;
+            const int nwb = min(NW, ns - b * NW);
             Stripe<R> sp;
             load_stripe<R>(a, P, s, lane, sp);
             float qn[7];                             /* transitions of the row below the lane's last row */
@@ -328,152 +412,191 @@ __global__ __launch_bounds__(256) void k_sdp_adj(SdpArgs a)
             for (int k = 0; k < R; k++) cur[k] = s5_fill(NEG, NEG);
             Up3 sh, shp;
             sh.bs = sh.mi = sh.dn = NEG; shp = sh;
-            if (lane == 63 && below) { sh.bs = bh[0 * L2 + L2 - 1]; sh.mi = bh[1 * L2 + L2 - 1]; sh.dn = bh[2 * L2 + L2 - 1]; }
             float dx[7];                             /* transitions of column c+1 */
             for (int j = 0; j < 7; j++) dx[j] = 0.0f;
-            const int nseg = (a.K > 0) ? (nsteps + a.K - 1) / a.K : 1;
-            for (int gseg = nseg - 1; gseg >= 0; gseg--) {
-                const int t0 = (a.K > 0) ? gseg * a.K : 0;
-                const int t1 = (a.K > 0) ? min(nsteps, t0 + a.K) : nsteps;
-                if (a.K > 0) {
-                    /* recompute the segment's forward states from its checkpoint */
+            const int nph = nsegp + lag * (nwb - 1);
+            for (int ph = 0; ph < nph; ph++) {
+                const int gseg = nsegp - 1 - ph + lag * (nwb - 1 - wv);   /* this wave's phase */
+                const bool segok = have && gseg >= 0 && gseg < nsegp;
+                const int t0 = gseg * KK;
+                if (a.K > 0 && segok && t0 < nsteps) {
+                    /* recompute the phase's forward states from its checkpoint */
+                    const int t1 = min(nsteps, t0 + KK);
                     FwdState<R> w;
-                    if (gseg == 0) fwd_init<R>(a, P, fh, s, lane, w);
-                    else get_ckpt<R>(store + ((int64_t)s * P.ncp + gseg) * ckpt_floats<R>(), lane, w);
-                    float zm = 0.0f, zs = 0.0f;
+                    if (gseg == 0) {
+#pragma unroll
+                        for (int k = 0; k < R; k++) w.H[k] = s5_fill(NEG, NEG);
+                        w.D = s5_fill(NEG, NEG);
+                        w.U = (lane == 0) ? top_at(a, P, fh, s, 0) : s5_fill(NEG, NEG);
+                    } else {
+                        get_ckpt<R>(store + ((int64_t)s * P.ncp + gseg) * ckpt_floats<R>(), lane, w);
+                    }
                     for (int t = t0; t < t1; t++) {
-                        fwd_step<R>(a, P, sp, w, s, ns, t, lane, th, S, fh, nullptr, zm, zs, false);
+                        fwd_step<R>(a, P, sp, w, s, ns, t, lane, th, S, fh);
                         if ((t - lane) >= 0 && (t - lane) < L2)
                             put_cells<R>(scr + (int64_t)(t - t0) * 5 * R * 64, lane, w.H);
                     }
                     __threadfence_block();
                 }
-                for (int t = t1 - 1; t >= t0; t--) {
-                    const int c = t - lane;
-                    const bool act = (c >= 0) && (c < L2);
-                    float dc[7];
-                    if (act) ld_trn(a.dtr, a.dntr, db, c, dc);
-                    else for (int j = 0; j < 7; j++) dc[j] = 0.0f;
-                    const float* fb = (a.K > 0) ? scr + (int64_t)(t - t0) * 5 * R * 64
-                                                : store + ((int64_t)s * nsteps + t) * 5 * R * 64;
-                    if (act) {
-                        S5 nxt = s5_fill(NEG, NEG);   /* new values of row k+1 (current column) */
-                        float oldbs = shp.bs;         /* B_MM+S of row k+1 at column c+1 */
+                for (int j = 0; j < KK; j++) {
+                    const int T = ph * KK + j;
+                    const int t = t0 + KK - 1 - j;
+                    if (segok && t < nsteps) {
+                        const int c = t - lane;
+                        const bool act = (c >= 0) && (c < L2);
+                        float dc[7];
+                        if (act) ld_trn(a.dtr, a.dntr, db, c, dc);
+                        else for (int jj = 0; jj < 7; jj++) dc[jj] = 0.0f;
+                        const float* fb = (a.K > 0) ? scr + (int64_t)(t - t0) * 5 * R * 64
+                                                    : store + ((int64_t)s * nsteps + t) * 5 * R * 64;
+                        if (act) {
+                            S5 nxt = s5_fill(NEG, NEG);   /* new values of row k+1 (current column) */
+                            float oldbs = shp.bs;         /* B_MM+S of row k+1 at column c+1 */
 #pragma unroll
-                        for (int k = R - 1; k >= 0; k--) {
-                            const S5 rt = cur[k];                               /* (r, c+1) */
-                            const float dgbs = oldbs;                           /* (r+1, c+1) */
-                            oldbs = rt.mm;
-                            const float dnmi = (k < R - 1) ? nxt.mi : sh.mi;    /* (r+1, c) */
-                            const float dndn = (k < R - 1) ? nxt.dn : sh.dn;
-                            const float* qs = sp.qt[k];
-                            const float* qr = (k < R - 1) ? sp.qt[k + 1] : qn;  /* row r+1 */
-                            S5 nb = s5_fill(NEG, NEG);
-                            if (sp.valid[k]) {
-                                /* MM: end, MM->MM, MM->MI, MM->IM, MM->DN, MM->ND */
-                                const float m1 = dgbs + qr[0] + dx[0];
-                                const float m2 = dnmi + qr[0] + dc[1] + th.oi;
-                                const float m3 = rt.im + qs[1] + dx[0] + th.oi;
-                                const float m4 = dndn + qr[2] + th.od;
-                                const float m5 = rt.nd + dx[2] + th.od;
-                                const float mm = fmaxf(fmaxf(fmaxf(m1, m2), fmaxf(m3, m4)), fmaxf(m5, 0.0f));
-                                const float e2 = ex2(m2 - mm), e3 = ex2(m3 - mm), e4 = ex2(m4 - mm), e5 = ex2(m5 - mm);
-                                const float bmm = mm + lg2(ex2(-mm) + ex2(m1 - mm) + e2 + e3 + e4 + e5);
-                                /* MI: close, extend */
-                                const float i1 = dgbs + qr[0] + dx[3] + th.cl, i2 = dnmi + qr[0] + dc[4] + th.ei;
-                                const float mi = fmaxf(i1, i2), fi1 = ex2(i1 - mi), fi2 = ex2(i2 - mi);
-                                const float bmi = mi + lg2(fi1 + fi2);
-                                /* IM */
-                                const float j1 = dgbs + qr[3] + dx[0] + th.cl, j2 = rt.im + qs[4] + dx[0] + th.ei;
-                                const float mj = fmaxf(j1, j2), fj1 = ex2(j1 - mj), fj2 = ex2(j2 - mj);
-                                const float bim = mj + lg2(fj1 + fj2);
-                                /* DN */
-                                const float d1 = dgbs + qr[5] + dx[0] + th.cl, d2 = dndn + qr[6] + th.ed;
-                                const float md = fmaxf(d1, d2), fd1 = ex2(d1 - md), fd2 = ex2(d2 - md);
-                                const float bdn = md + lg2(fd1 + fd2);
-                                /* ND */
-                                const float n1 = dgbs + qr[0] + dx[5] + th.cl, n2 = rt.nd + dx[6] + th.ed;
-                                const float mn = fmaxf(n1, n2), fn1 = ex2(n1 - mn), fn2 = ex2(n2 - mn);
-                                const float bnd = mn + lg2(fn1 + fn2);
-                                S5 f;
-                                get_cell<R>(fb, lane, k, f);
-                                const size_t ci = (size_t)sp.r[k] * L2 + c;
-                                const float wz = ex2(bmm - Z);
-                                dS[ci] = (ex2(f.mm + bmm - Z) - wz) * g;
-                                if (post) post[ci] = ex2(f.mm + bmm - Z);
-                                const float wm = ex2(f.mm + mm - Z), wi = ex2(f.mi + mi - Z), wj = ex2(f.im + mj - Z);
-                                const float wd = ex2(f.dn + md - Z), wn = ex2(f.nd + mn - Z);
-                                acc0 += (double)(wm * (e2 + e3));
-                                acc2 += (double)(wm * (e4 + e5));
-                                acc1 += (double)(wi * fi2 + wj * fj2);
-                                acc3 += (double)(wd * fd2 + wn * fn2);
-                                acc4 += (double)(wi * fi1 + wj * fj1 + wd * fd1 + wn * fn1);
-                                const float sc = S[ci] * L2E;
-                                if (c == 0 || sp.r[k] == 0) {
-                                    /* transitions whose source is the row -1 / column -1
-                                     * boundary: counted here at their destination */
-                                    const float lp0 = -99.0f * L2E;
-                                    const S5 up0 = s5_fill(0.0f, lp0);
-                                    S5 lf0 = up0, dg0 = up0;
-                                    if (c == 0) {
-                                        const int64_t bi = ((int64_t)a.qdist[P.q] + sp.r[k]) * 5;
-                                        lf0.mm = a.qll[bi] * L2E; lf0.mi = a.qll[bi + 1] * L2E; lf0.im = a.qll[bi + 2] * L2E;
-                                        lf0.dn = a.qll[bi + 3] * L2E; lf0.nd = a.qll[bi + 4] * L2E;
-                                        dg0.mm = a.qld[bi] * L2E; dg0.mi = a.qld[bi + 1] * L2E; dg0.im = a.qld[bi + 2] * L2E;
-                                        dg0.dn = a.qld[bi + 3] * L2E; dg0.nd = a.qld[bi + 4] * L2E;
-                                        acc0 += (double)ex2(lf0.mm + qs[1] + dc[0] + th.oi + bim - Z);
-                                        acc1 += (double)ex2(lf0.im + qs[4] + dc[0] + th.ei + bim - Z);
-                                        acc2 += (double)ex2(lf0.mm + dc[2] + th.od + bnd - Z);
-                                        acc3 += (double)ex2(lf0.nd + dc[6] + th.ed + bnd - Z);
+                            for (int k = R - 1; k >= 0; k--) {
+                                const S5 rt = cur[k];                               /* (r, c+1) */
+                                const float dgbs = oldbs;                           /* (r+1, c+1) */
+                                oldbs = rt.mm;
+                                const float dnmi = (k < R - 1) ? nxt.mi : sh.mi;    /* (r+1, c) */
+                                const float dndn = (k < R - 1) ? nxt.dn : sh.dn;
+                                const float* qs = sp.qt[k];
+                                const float* qr = (k < R - 1) ? sp.qt[k + 1] : qn;  /* row r+1 */
+                                S5 nb = s5_fill(NEG, NEG);
+                                if (sp.valid[k]) {
+                                    /* MM: end, MM->MM, MM->MI, MM->IM, MM->DN, MM->ND */
+                                    const float m1 = dgbs + qr[0] + dx[0];
+                                    const float m2 = dnmi + qr[0] + dc[1] + th.oi;
+                                    const float m3 = rt.im + qs[1] + dx[0] + th.oi;
+                                    const float m4 = dndn + qr[2] + th.od;
+                                    const float m5 = rt.nd + dx[2] + th.od;
+                                    const float mm = fmaxf(fmaxf(fmaxf(m1, m2), fmaxf(m3, m4)), fmaxf(m5, 0.0f));
+                                    const float e2 = ex2(m2 - mm), e3 = ex2(m3 - mm), e4 = ex2(m4 - mm), e5 = ex2(m5 - mm);
+                                    const float sum6 = ex2(-mm) + ex2(m1 - mm) + e2 + e3 + e4 + e5;
+                                    const float bmm = mm + lg2(sum6);
+                                    /* gap states: two terms each, the larger one is e^0 = 1 */
+                                    float fi1, fi2, fj1, fj2, fd1, fd2, fn1, fn2;
+                                    /* MI: close, extend */
+                                    const float i1 = dgbs + qr[0] + dx[3] + th.cl, i2 = dnmi + qr[0] + dc[4] + th.ei;
+                                    const float mi = lse_parts(i1, i2, fi1, fi2);
+                                    const float bmi = mi + lg2(fi1 + fi2);
+                                    /* IM */
+                                    const float j1 = dgbs + qr[3] + dx[0] + th.cl, j2 = rt.im + qs[4] + dx[0] + th.ei;
+                                    const float mj = lse_parts(j1, j2, fj1, fj2);
+                                    const float bim = mj + lg2(fj1 + fj2);
+                                    /* DN */
+                                    const float d1 = dgbs + qr[5] + dx[0] + th.cl, d2 = dndn + qr[6] + th.ed;
+                                    const float md = lse_parts(d1, d2, fd1, fd2);
+                                    const float bdn = md + lg2(fd1 + fd2);
+                                    /* ND */
+                                    const float n1 = dgbs + qr[0] + dx[5] + th.cl, n2 = rt.nd + dx[6] + th.ed;
+                                    const float mn = lse_parts(n1, n2, fn1, fn2);
+                                    const float bnd = mn + lg2(fn1 + fn2);
+                                    S5 f;
+                                    get_cell<R>(fb, lane, k, f);
+                                    const size_t ci = (size_t)sp.r[k] * L2 + c;
+                                    const float wz = ex2(bmm - Z);
+                                    /* e^{F_MM + B_MM - Z} = e^{F_MM + mm - Z} * sum6 */
+                                    const float wm = ex2(f.mm + mm - Z), wi = ex2(f.mi + mi - Z), wj = ex2(f.im + mj - Z);
+                                    const float pmm = wm * sum6;
+                                    dS[ci] = (pmm - wz) * g;
+                                    if (post) post[ci] = pmm;
+                                    const float wd = ex2(f.dn + md - Z), wn = ex2(f.nd + mn - Z);
+                                    acc0 += (double)(wm * (e2 + e3));
+                                    acc2 += (double)(wm * (e4 + e5));
+                                    acc1 += (double)(wi * fi2 + wj * fj2);
+                                    acc3 += (double)(wd * fd2 + wn * fn2);
+                                    acc4 += (double)(wi * fi1 + wj * fj1 + wd * fd1 + wn * fn1);
+                                    const float sc = S[ci] * L2E;
+                                    if (c == 0 || sp.r[k] == 0) {
+                                        /* transitions whose source is the row -1 / column -1
+                                         * boundary: counted here at their destination */
+                                        const float lp0 = -99.0f * L2E;
+                                        const S5 up0 = s5_fill(0.0f, lp0);
+                                        S5 lf0 = up0, dg0 = up0;
+                                        if (c == 0) {
+                                            const int64_t bi = ((int64_t)a.qdist[P.q] + sp.r[k]) * 5;
+                                            lf0.mm = a.qll[bi] * L2E; lf0.mi = a.qll[bi + 1] * L2E; lf0.im = a.qll[bi + 2] * L2E;
+                                            lf0.dn = a.qll[bi + 3] * L2E; lf0.nd = a.qll[bi + 4] * L2E;
+                                            dg0.mm = a.qld[bi] * L2E; dg0.mi = a.qld[bi + 1] * L2E; dg0.im = a.qld[bi + 2] * L2E;
+                                            dg0.dn = a.qld[bi + 3] * L2E; dg0.nd = a.qld[bi + 4] * L2E;
+                                            acc0 += (double)ex2(lf0.mm + qs[1] + dc[0] + th.oi + bim - Z);
+                                            acc1 += (double)ex2(lf0.im + qs[4] + dc[0] + th.ei + bim - Z);
+                                            acc2 += (double)ex2(lf0.mm + dc[2] + th.od + bnd - Z);
+                                            acc3 += (double)ex2(lf0.nd + dc[6] + th.ed + bnd - Z);
+                                        }
+                                        if (sp.r[k] == 0) {
+                                            acc0 += (double)ex2(up0.mm + qs[0] + dc[1] + th.oi + bmi - Z);
+                                            acc1 += (double)ex2(up0.mi + qs[0] + dc[4] + th.ei + bmi - Z);
+                                            acc2 += (double)ex2(up0.mm + qs[2] + th.od + bdn - Z);
+                                            acc3 += (double)ex2(up0.dn + qs[6] + th.ed + bdn - Z);
+                                        }
+                                        acc4 += (double)(ex2(dg0.mi + qs[0] + dc[3] + th.cl + sc + bmm - Z) +
+                                                         ex2(dg0.im + qs[3] + dc[0] + th.cl + sc + bmm - Z) +
+                                                         ex2(dg0.dn + qs[5] + dc[0] + th.cl + sc + bmm - Z) +
+                                                         ex2(dg0.nd + qs[0] + dc[5] + th.cl + sc + bmm - Z));
                                     }
-                                    if (sp.r[k] == 0) {
-                                        acc0 += (double)ex2(up0.mm + qs[0] + dc[1] + th.oi + bmi - Z);
-                                        acc1 += (double)ex2(up0.mi + qs[0] + dc[4] + th.ei + bmi - Z);
-                                        acc2 += (double)ex2(up0.mm + qs[2] + th.od + bdn - Z);
-                                        acc3 += (double)ex2(up0.dn + qs[6] + th.ed + bdn - Z);
-                                    }
-                                    acc4 += (double)(ex2(dg0.mi + qs[0] + dc[3] + th.cl + sc + bmm - Z) +
-                                                     ex2(dg0.im + qs[3] + dc[0] + th.cl + sc + bmm - Z) +
-                                                     ex2(dg0.dn + qs[5] + dc[0] + th.cl + sc + bmm - Z) +
-                                                     ex2(dg0.nd + qs[0] + dc[5] + th.cl + sc + bmm - Z));
+                                    nb.mm = bmm + sc;
+                                    nb.mi = bmi; nb.im = bim; nb.dn = bdn; nb.nd = bnd;
                                 }
-                                nb.mm = bmm + sc;
-                                nb.mi = bmi; nb.im = bim; nb.dn = bdn; nb.nd = bnd;
+                                cur[k] = nb;
+                                nxt = nb;
                             }
-                            cur[k] = nb;
-                            nxt = nb;
-                        }
-                        if (lane == 0 && s > 0) {
-                            bh[0 * L2 + c] = cur[0].mm; bh[1 * L2 + c] = cur[0].mi; bh[2 * L2 + c] = cur[0].dn;
-                        }
-                    } else {
+                            if (lane == 0) {
+                                if (wv > 0) {
+                                    float* o = hb[T & 1][wv];
+                                    o[0] = cur[0].mm; o[1] = cur[0].mi; o[2] = cur[0].dn;
+                                } else if (s > 0) {
+                                    bh[0 * L2 + c] = cur[0].mm; bh[1 * L2 + c] = cur[0].mi; bh[2 * L2 + c] = cur[0].dn;
+                                }
+                            }
+                        } else {
 #pragma unroll
-                        for (int k = 0; k < R; k++) cur[k] = s5_fill(NEG, NEG);
+                            for (int k = 0; k < R; k++) cur[k] = s5_fill(NEG, NEG);
+                        }
+                        for (int jj = 0; jj < 7; jj++) dx[jj] = dc[jj];
                     }
-                    for (int j = 0; j < 7; j++) dx[j] = dc[j];
-                    /* row below for the next step (t-1): lane l+1's first row, or the
-                     * stripe below at column t-1-63 for lane 63 */
-                    Up3 ob; ob.bs = ob.mi = ob.dn = NEG;
-                    const int cb = t - 1 - 63;
-                    if (lane == 63 && below && cb >= 0 && cb < L2) { ob.bs = bh[0 * L2 + cb]; ob.mi = bh[1 * L2 + cb]; ob.dn = bh[2 * L2 + cb]; }
-                    shp = sh;
-                    sh.bs = wave_shl1(cur[0].mm, ob.bs);
-                    sh.mi = wave_shl1(cur[0].mi, ob.mi);
-                    sh.dn = wave_shl1(cur[0].dn, ob.dn);
+                    __syncthreads();
+                    if (segok && t >= 1 && t <= nsteps) {
+                        /* the row below for step t-1: lane l+1's first row, or for
+                         * lane 63 the stripe below at column t-1-63 */
+                        Up3 ob; ob.bs = ob.mi = ob.dn = NEG;
+                        const int cb = t - 1 - 63;
+                        if (lane == 63 && below && cb >= 0 && cb < L2) {
+                            if (wv + 1 < nwb) {
+                                const float* o = hb[T & 1][wv + 1];
+                                ob.bs = o[0]; ob.mi = o[1]; ob.dn = o[2];
+                            } else {
+                                ob.bs = bh[0 * L2 + cb]; ob.mi = bh[1 * L2 + cb]; ob.dn = bh[2 * L2 + cb];
+                            }
+                        }
+                        shp = sh;
+                        sh.bs = wave_shl1(cur[0].mm, ob.bs);
+                        sh.mi = wave_shl1(cur[0].mi, ob.mi);
+                        sh.dn = wave_shl1(cur[0].dn, ob.dn);
+                    }
                 }
             }
-            __threadfence();
+            __syncthreads();
         }
-        /* dtheta: fixed-order xor-butterfly in double */
+        /* dtheta: fixed-order xor-butterfly in double, then the waves in order */
         for (int off = 32; off >= 1; off >>= 1) {
             acc0 += __shfl_xor(acc0, off); acc1 += __shfl_xor(acc1, off); acc2 += __shfl_xor(acc2, off);
             acc3 += __shfl_xor(acc3, off); acc4 += __shfl_xor(acc4, off);
         }
         if (lane == 0) {
-            float* d = a.dth + (size_t)pi * 5;
-            d[0] = (float)(acc0 * g); d[1] = (float)(acc1 * g); d[2] = (float)(acc2 * g);
-            d[3] = (float)(acc3 * g); d[4] = (float)(acc4 * g);
+            wacc[wv][0] = acc0; wacc[wv][1] = acc1; wacc[wv][2] = acc2; wacc[wv][3] = acc3; wacc[wv][4] = acc4;
         }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double t0 = 0.0, t1 = 0.0, t2 = 0.0, t3 = 0.0, t4 = 0.0;
+            for (int k = 0; k < NW; k++) {
+                t0 += wacc[k][0]; t1 += wacc[k][1]; t2 += wacc[k][2]; t3 += wacc[k][3]; t4 += wacc[k][4];
+            }
+            float* d = a.dth + (size_t)pi * 5;
+            d[0] = (float)(t0 * g); d[1] = (float)(t1 * g); d[2] = (float)(t2 * g);
+            d[3] = (float)(t3 * g); d[4] = (float)(t4 * g);
+        }
+        __syncthreads();
     }
 }
 
@@ -522,12 +645,13 @@ int c2_sdp_rows_per_lane() { return 5; }
 
 int64_t c2_sdp_ckpt_floats(int R) { return (int64_t)(5 * R + 10) * 64; }
 
-hipError_t c2_launch_sdp(const SdpArgs& a, bool adjoint, int nwaves, hipStream_t stream)
+hipError_t c2_launch_sdp(const SdpArgs& a, bool adjoint, int groups, int nw, hipStream_t stream)
 {
     if (a.npairs <= 0) return hipSuccess;
-    const int blocks = (nwaves + 3) / 4;
-    if (adjoint) hipLaunchKernelGGL((k_sdp_adj<5>), dim3(blocks), dim3(256), 0, stream, a);
-    else hipLaunchKernelGGL((k_sdp_fwd<5>), dim3(blocks), dim3(256), 0, stream, a);
+    if (groups < 1 || nw < 1 || nw > 8) return hipErrorInvalidValue;
+    if (a.K < 0 || (a.K > 0 && 64 % a.K != 0)) return hipErrorInvalidValue;
+    if (adjoint) hipLaunchKernelGGL((k_sdp_adj<5>), dim3(groups), dim3(64 * nw), 0, stream, a);
+    else hipLaunchKernelGGL((k_sdp_fwd<5>), dim3(groups), dim3(64 * nw), 0, stream, a);
     return hipGetLastError();
 }
 

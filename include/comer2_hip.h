@@ -191,7 +191,8 @@ typedef struct {
 } cbp_sdp_pair;
 /* plan the pairs; ckpt = 0 keeps every forward cell for the adjoint, K > 0
  * keeps a checkpoint every K anti-diagonal steps and recomputes, -1 picks
- * K = ceil(sqrt(max steps)); returns the workspace size the caller allocates */
+ * K = ceil(sqrt(max steps)); K is rounded down to 16, 32 or 64 (at least 16);
+ * returns the workspace size the caller allocates */
 int  cbp_sdp_prepare(cbp_handle* h, const cbp_sdp_pair* pairs, int32_t npairs, int32_t ckpt,
                      int64_t* ws_bytes);
 /* score matrices of the prepared pairs into S (device), the score of

@@ -19,16 +19,22 @@ def _strings(aln):
     return tuple(bytes(aln[j]).split(b"\0")[0].decode() for j in range(3))
 
 
-def _planted_sets(seed):
-    q = SynthDB(seed, 2, 150, 0.668, 60, 400)
+def _planted_sets(seed, qlen=None):
+    if qlen is None:
+        q = SynthDB(seed, 2, 150, 0.668, 60, 400)
+    else:
+        q = SynthDB(seed, 1, 0, 0, qlen[0], qlen[1])
     d = SynthDB(seed + 1, 90, 160, 0.668, 40, 600)
     d.plant(q, seed + 2, 14, 3.0, 60.0)
     return q, d
 
 
-@pytest.mark.parametrize("seed,cap", [(41, 0), (51, 5)])
-def test_map_matches_oracle(seed, cap):
-    q, d = _planted_sets(seed)
+# query lengths 60-400: one band of 1-7 stripes of 64 rows; 560-640: stripes
+# of 128 rows; 2600-2700: 320-row stripes in two bands (HBM band hand-off)
+@pytest.mark.parametrize("seed,cap,qlen", [(41, 0, None), (51, 5, None), (61, 6, (560, 640)),
+                                           (71, 4, (2600, 2700))])
+def test_map_matches_oracle(seed, cap, qlen):
+    q, d = _planted_sets(seed, qlen)
     orc = oracle()
     cbp = processing(dict(max_map_pairs=cap))
     cbp.TransferQueries(q)
@@ -37,7 +43,7 @@ def test_map_matches_oracle(seed, cap):
     logev = float(np.log(1e-3))
     total = aligned = 0
     for qp in range(q.nprof):
-        print(f"map parity: seed={seed} cap={cap} query={qp}", flush=True)
+        print(f"map parity: seed={seed} cap={cap} query={qp} len={q.lengths()[qp]}", flush=True)
         res = cbp.ProcessScoreMatrix(qp, scorethld=4.0, logevthld=logev)
         order = res["order"]
         sel = [int(i) for i in order if res["logE"][i] <= logev]

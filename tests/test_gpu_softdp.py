@@ -58,6 +58,41 @@ def test_soft_dp_matches_fp64_oracle(ckpt):
         assert np.all(np.abs(dth[k] - eth) <= 1e-3 * np.maximum(1.0, np.abs(eth))), (ctx, dth[k], eth)
 
 
+@pytest.mark.parametrize("ckpt", [0, 32])
+def test_soft_dp_two_bands(ckpt):
+    """A 2,600-2,700-row query is nine 320-row stripes: two bands of waves,
+    the second fed through the HBM hand-off."""
+    import torch
+    from comer2_amd.softdp import SoftDP
+    q = SynthDB(91, 1, 0, 0, 2600, 2700)
+    d = SynthDB(92, 3, 0, 0, 40, 90)
+    cbp = processing()
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    pairs = [(0, dp) for dp in range(d.nprof)]
+    print(f"soft dp two bands: ckpt={ckpt} lengths q={q.lengths().tolist()} d={d.lengths().tolist()}", flush=True)
+    orc = oracle()
+    sdp = SoftDP(cbp, pairs, ckpt=ckpt)
+    Sh = _oracle_scores(orc, q, d, sdp)
+    S = torch.from_numpy(Sh).cuda()
+    th = torch.tensor([0.1, -0.2, 0.05, -0.1, 0.1], dtype=torch.float32, device="cuda")
+    logZ = sdp.forward(S, th)
+    dS, dth, post = sdp.adjoint(S, th, logZ, want_post=True)
+    torch.cuda.synchronize()
+    logZ, dS, dth, post = logZ.cpu().numpy(), dS.cpu().numpy(), dth.cpu().numpy(), post.cpu().numpy()
+    thd = th.cpu().numpy().astype(np.float64)
+    for k, (qp, dp) in enumerate(sdp.pairs):
+        L1, L2 = sdp.shapes[k]
+        o = int(sdp.offsets[k])
+        Sk = Sh[o:o + L1 * L2].reshape(L1, L2).astype(np.float64)
+        z, eS, eth, epost = orc.sdp_grad(q.view, qp, d.view, dp, Sk, thd)
+        ctx = (ckpt, qp, dp, L1, L2)
+        assert abs(logZ[k] - z) <= 1e-5 * abs(z), ctx
+        assert np.max(np.abs(dS[o:o + L1 * L2] - eS.ravel())) < 1e-4, ctx
+        assert np.max(np.abs(post[o:o + L1 * L2] - epost.ravel())) < 1e-4, ctx
+        assert np.all(np.abs(dth[k] - eth) <= 1e-3 * np.maximum(1.0, np.abs(eth))), (ctx, dth[k], eth)
+
+
 def test_soft_dp_gradients_against_finite_differences():
     import torch
     from comer2_amd.softdp import SoftDP
