@@ -238,10 +238,11 @@ def main():
     q = SynthDB(QUERY_SEED, 1, QUERY_LEN, 0.0, QUERY_LEN, QUERY_LEN)
     d.plant(q, PLANT_SEED, N_PLANT if args.nprof == N_PROFILES else max(1, args.nprof // 50),
             PLANT_KMIN, PLANT_KMAX)
-    if not args.no_batch:
-        qb = SynthDB(3, 64, 0, 0, 50, 1000, sort_desc=False)
-        d.plant(qb, PLANT_SEED + 1, 20 * 64 if args.nprof == N_PROFILES else max(64, args.nprof // 100),
-                PLANT_KMIN, PLANT_KMAX)
+    # homologs of the 64 batch queries are planted whether or not the batch
+    # leg runs, so the one-query workload is the same DB either way
+    qb = SynthDB(3, 64, 0, 0, 50, 1000, sort_desc=False)
+    d.plant(qb, PLANT_SEED + 1, 20 * 64 if args.nprof == N_PROFILES else max(64, args.nprof // 100),
+            PLANT_KMIN, PLANT_KMAX)
     log("bench: generated %d profiles, %d positions in %.1f s" % (d.nprof, d.npos, time.perf_counter() - t))
 
     cbp = CuBatchProcessing(0, 0, 0, options=dict(max_map_pairs=MAP_CAP))

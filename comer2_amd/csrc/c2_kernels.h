@@ -21,6 +21,7 @@ struct P1Args {
     const float* lleft;   /* [5][Lpad]  column -1 left boundary */
     const float* ldiag;   /* [5][Lpad]  column -1 diagonal boundary */
     const float* qbkg;    /* [20] query background */
+    const float* wts;     /* [nprof][20] 1/(0.5 (qbkg + bkg of the profile)) */
     int32_t Lq, Lpad, nstripes;
     float shift, ssswgt;
     /* segments of whole profiles */
@@ -91,6 +92,9 @@ int64_t c2_map_hand(int nstripes, int Ld);
 
 size_t c2_phase1_lds_bytes(int R, int NW, int nstripes);
 hipError_t c2_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t stream);
+/* score weights of the query against every chunk profile into wts [nprof][20] */
+hipError_t c2_launch_weights(const float* qbkg, const float* bkg, int nprof, float* wts, hipStream_t stream);
+hipError_t c2_launch_zero_u32(uint32_t* p, int64_t n, hipStream_t stream);
 hipError_t c2_launch_finalize(const ScoreCtx& c, const int32_t* cand, int32_t ncand,
                               const unsigned long long* keys, const uint32_t* btck,
                               const int64_t* btck_off, FinRec* out, hipStream_t stream);

@@ -316,6 +316,9 @@ int c2_run_phase1(cbp_handle* h, int qidx)
         if ((rc = dalloc(h, h->bot, sizeof(float) * 5 * (size_t)h->dchunk.npos))) return rc;
         a.bot = h->bot.as<float>();
     }
+    if ((rc = dalloc(h, h->wts, sizeof(float) * 20 * (size_t)h->dchunk.nprof))) return rc;
+    a.wts = h->wts.as<float>();
+    HIPCHK(c2_launch_weights(dq->qbkg.as<float>(), h->dchunk.bkg.as<float>(), h->dchunk.nprof, h->wts.as<float>(), h->stream));
     int nw = h->nw;
     while (nw > 1 && c2_phase1_lds_bytes(dq->hp.R, nw, dq->hp.nstripes) > 160 * 1024) nw >>= 1;
     HIPCHK(hipMemsetAsync(h->keys.p, 0, sizeof(unsigned long long) * h->dchunk.nprof, h->stream));
@@ -527,10 +530,12 @@ extern "C" int cbp_sdp_prepare(cbp_handle* h, const cbp_sdp_pair* pairs, int32_t
         const SdpPair& A = h->sdp_pairs[a]; const SdpPair& B = h->sdp_pairs[b];
         return (int64_t)A.ns * (A.L2 + 63) > (int64_t)B.ns * (B.L2 + 63);
     });
-    /* workgroups of up to 8 waves (one stripe each), about 4096 waves in all */
+    /* workgroups of up to 8 waves (one stripe each), about 4096 waves in
+     * all: a stripe per wave only when the pairs alone do not fill the chip */
     int maxns = 1;
     for (const SdpPair& P : h->sdp_pairs) maxns = std::max(maxns, P.ns);
-    const int nw = std::min(maxns, 8);
+    const int want = npairs > 0 ? (4096 + npairs - 1) / npairs : 1;
+    const int nw = std::max(1, std::min(std::min(maxns, 8), want));
     const int groups = std::max(1, std::min(npairs, 4096 / nw));
     h->sdp_nw = nw;
     h->sdp_groups = groups;

@@ -53,6 +53,7 @@ struct QWork {
     size_t hkeys_n = 0;
     std::vector<int32_t> cand, pe;
     std::vector<int64_t> off;
+    DBuf wts;                           /* [nprof][20] score weights of the query */
     DBuf dcand, dpe, doff, btck, fin;
     std::vector<FinRec> hfin;
     std::vector<int32_t> pass;
@@ -110,6 +111,7 @@ struct cbp_handle {
     int32_t nseg = 0;
     DBuf keys;           /* [nprof] u64 */
     DBuf bot;            /* stripe hand-off */
+    DBuf wts;            /* [nprof][20] score weights of the query being searched */
     uint64_t dblen = 0, nseqs = 0;
     int seg_cols = 4096; /* tuning: target columns per segment */
     int rmax = 5;        /* tuning: max rows per lane */

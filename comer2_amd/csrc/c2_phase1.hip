@@ -29,7 +29,6 @@
  * BTCK=true additionally stores 2-bit backtrack codes packed per (step, lane)
  * for the pairs that pass to the finalize stage.
  */
-#include <type_traits>
 #include "c2_dev.h"
 #include "c2_kernels.h"
 
@@ -43,37 +42,62 @@ constexpr int NF_BASE = 33, NF_TOP = 38;
 
 /* tops: the ring carries the stripe's top boundary; rtop: read it from the
  * hand-off buffer (stripes after the first) instead of the zero row -1 */
+/* the profile of the column lane loads in the refill of block t0 (loaded
+ * half a block ahead, so the refill's loads do not wait on it) */
+__device__ __forceinline__ int block_profile(const P1Args& a, int lane, int t0, int64_t c_begin, int64_t c_end)
+{
+    const int64_t c = c_begin + t0 + lane;
+    return (lane < BLK && c < c_end) ? a.addrpro[c] : -1;
+}
+
 template <int R>
 __device__ __forceinline__ void load_block(const P1Args& a, float* ring, int lane, int t0,
-                                           int64_t c_begin, int64_t c_end, bool tops, bool rtop)
+                                           int64_t c_begin, int64_t c_end, bool tops, bool rtop, int p)
 {
     if (lane >= BLK) return;
     const int slot = (t0 + lane) % RING;
     const int64_t c = c_begin + t0 + lane;
     if (c < c_end) {
-        const int p = a.addrpro[c];
         const int x = (int)(c - (int64_t)a.dist[p]);
-        for (int k = 0; k < 20; k++) {
-            float w = 1.0f / (0.5f * (a.qbkg[k] + a.bkg[(size_t)k * a.nprof + p]));
-            ring[(F_DW + k) * RING + slot] = a.trgfrq[(size_t)k * a.npos + c] * w;
+        /* the field stride is made opaque here so that the 20 per-field base
+         * addresses are formed per refill instead of being held in registers
+         * across the step loop */
+        int64_t npos = a.npos;
+        asm volatile("" : "+s"(npos));
+        const float* tf = a.trgfrq + c;
+        if constexpr (R >= 5) {
+            /* the widest kernel keeps its registers for the rows: the weights
+             * are formed here (the same FP32 operations as the table) */
+            for (int k = 0; k < 20; k++) {
+                const float w = 1.0f / (0.5f * (a.qbkg[k] + a.bkg[(size_t)k * a.nprof + p]));
+                ring[(F_DW + k) * RING + slot] = a.trgfrq[(size_t)k * a.npos + c] * w;
+            }
+        } else {
+            const float* wp = a.wts + (size_t)p * 20;
+            for (int k = 0; k < 20; k++)
+                ring[(F_DW + k) * RING + slot] = tf[(size_t)k * npos] * wp[k];
         }
-        const size_t ntr = (size_t)(a.npos + a.nprof);
-        const size_t tb = (size_t)c + p;     /* row x of profile p: dist + p + x */
-        ring[(F_DT + 0) * RING + slot] = a.trans[0 * ntr + tb];
-        ring[(F_DT + 1) * RING + slot] = a.trans[1 * ntr + tb + 1];
-        ring[(F_DT + 2) * RING + slot] = a.trans[2 * ntr + tb];
-        ring[(F_DT + 3) * RING + slot] = a.trans[3 * ntr + tb];
-        ring[(F_DT + 4) * RING + slot] = a.trans[4 * ntr + tb + 1];
-        ring[(F_DT + 5) * RING + slot] = a.trans[5 * ntr + tb];
-        ring[(F_DT + 6) * RING + slot] = a.trans[6 * ntr + tb];
+        size_t ntr = (size_t)(a.npos + a.nprof);
+        asm volatile("" : "+s"(ntr));
+        const float* tr = a.trans + (size_t)c + p;     /* row x of profile p: dist + p + x */
+        ring[(F_DT + 0) * RING + slot] = tr[0 * ntr];
+        ring[(F_DT + 1) * RING + slot] = tr[1 * ntr + 1];
+        ring[(F_DT + 2) * RING + slot] = tr[2 * ntr];
+        ring[(F_DT + 3) * RING + slot] = tr[3 * ntr];
+        ring[(F_DT + 4) * RING + slot] = tr[4 * ntr + 1];
+        ring[(F_DT + 5) * RING + slot] = tr[5 * ntr];
+        ring[(F_DT + 6) * RING + slot] = tr[6 * ntr];
+        const float* sp = a.ssprb + c;
         for (int j = 0; j < 3; j++)
-            ring[(F_PJ + j) * RING + slot] = a.ssprb[(size_t)j * a.npos + c];
+            ring[(F_PJ + j) * RING + slot] = sp[(size_t)j * npos];
         ring[F_X * RING + slot] = __int_as_float(x);
         ring[F_LEN * RING + slot] = __int_as_float(a.len[p]);
         ring[F_PID * RING + slot] = __int_as_float(p);
-        if (tops)
+        if (tops) {
+            const float* bt = a.bot + c;
             for (int s = 0; s < 5; s++)
-                ring[(F_TOP + s) * RING + slot] = rtop ? a.bot[(size_t)s * a.npos + c] : 0.0f;
+                ring[(F_TOP + s) * RING + slot] = rtop ? bt[(size_t)s * npos] : 0.0f;
+        }
     } else {
         for (int k = 0; k < 20; k++) ring[(F_DW + k) * RING + slot] = 0.0f;
         for (int k = 0; k < 7; k++) ring[(F_DT + k) * RING + slot] = -32768.0f;
@@ -150,7 +174,8 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
             if (tops) for (int j = 0; j < 5; j++) ring[(F_TOP + j) * RING + i] = 0.0f;
         }
         wave_lds_sync();
-        load_block<R>(a, ring, lane, 0, c_begin, c_end, tops, s > 0);
+        int pn = block_profile(a, lane, 0, c_begin, c_end);
+        load_block<R>(a, ring, lane, 0, c_begin, c_end, tops, s > 0, pn);
         wave_lds_sync();
 
         S5 H[R];
@@ -167,60 +192,14 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
         const bool wbot = tops && (s + 1 < a.nstripes);
         uint32_t* btw = nullptr;
         if (BTCK) btw = a.btck + a.btck_off[seg] + (size_t)s * nsteps * 64;
-        const bool ssterm = a.ssswgt > 0.0f;
 
-        /* the R cells of one step; FIX: some lane is at a profile's column 0,
-         * where row kf takes a zero diagonal (the reference's 32-row block
-         * boundary) */
-        auto cells = [&](auto fixc, int t, bool first, const float* dw, const float* dt, const float* pj,
-                         int xloc) {
-            constexpr bool FIX = decltype(fixc)::value;
-            int co[3];
-#pragma unroll
-            for (int j = 0; j < 3; j++) co[j] = (10 * j + (int)(pj[j] * 9.999f)) * SL + R * lane;
-            const float dh = (pj[2] + 0.1f) * 0.1111111f;
-            const int u0 = (xloc - y0) - cA - 1;
-            uint32_t bw = 0;
-            S5 saved = D;
-#pragma unroll
-            for (int k = 0; k < R; k++) {
-                /* score: log(sum_a q_a d_a w_a) - shift + SS term */
-                float acc = q[k][0] * dw[0];
-#pragma unroll
-                for (int j = 1; j < 20; j++) acc = fmaf(q[k][j], dw[j], acc);
-                float sc = c2_logf_clamped(acc) - a.shift;
-                if (ssterm) {
-                    float ss = sQT[co[0] + k] * pj[0];
-                    ss = fmaf(sQT[co[1] + k], pj[1], ss);
-                    ss = fmaf(sQT[co[2] + k], pj[2], ss);
-                    ss = fmaf(qh[k], dh, ss);
-                    ss = (ss - 0.05f) * a.ssswgt;
-                    sc = sc + ss;
-                }
-                if ((uint32_t)(u0 - k) >= (uint32_t)cW) sc = -9.0f;   /* distant corners */
-                S5 dg = saved;
-                if (FIX && k > 0) {
-                    const bool z = first && kf == k;
-                    dg.mm = z ? 0.0f : dg.mm; dg.mi = z ? 0.0f : dg.mi; dg.im = z ? 0.0f : dg.im;
-                    dg.dn = z ? 0.0f : dg.dn; dg.nd = z ? 0.0f : dg.nd;
-                }
-                const S5 up = (k == 0) ? U : H[k - 1];
-                saved = H[k];
-                const S5 n = dp_cell(dg, up, H[k], qt[k], dt, sc);
-                H[k] = n;
-                if (n.mm > best) { best = n.mm; bpos = (t << 3) | k; }
-                if (BTCK) bw |= dp_btck(n) << (2 * k);
-            }
-            if (BTCK) btw[(size_t)t * 64 + lane] = bw;
-        };
-
-        /* one step t: ring refill, profile starts, cells, stripe hand-off,
-         * row above for step t+1 */
         for (int t = 0; t < nsteps; t++) {
             if (((t + 1) % BLK) == 0) {
-                load_block<R>(a, ring, lane, t + 1, c_begin, c_end, tops, s > 0);
+                load_block<R>(a, ring, lane, t + 1, c_begin, c_end, tops, s > 0,
+                              block_profile(a, lane, t + 1, c_begin, c_end));
                 wave_lds_sync();
             }
+
             const int slot = (t - lane + RING * 4) % RING;
             float dw[20], dt[7], pj[3];
 #pragma unroll
@@ -233,6 +212,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
 
             /* profile start: flush, reset to the column -1 boundary */
             const bool first = (xloc == 0);
+            uint32_t fixbits = 0;
             if (__builtin_amdgcn_read_exec() & __ballot(first)) {
                 if (first) {
                     if (curpid >= 0 && best > 0.0f) {
@@ -259,8 +239,48 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
                     D.mm = sLD[0 * SL + sl0]; D.mi = sLD[1 * SL + sl0]; D.im = sLD[2 * SL + sl0];
                     D.dn = sLD[3 * SL + sl0]; D.nd = sLD[4 * SL + sl0];
                 }
+#pragma unroll
+                for (int k = 1; k < R; k++)
+                    if (__ballot(first && kf == k)) fixbits |= 1u << k;
             }
-            cells(std::true_type{}, t, first, dw, dt, pj, xloc);
+
+            /* per-column SS table offsets: QT[10j + int(9.999 p'_j)] */
+            int co[3];
+#pragma unroll
+            for (int j = 0; j < 3; j++) co[j] = (10 * j + (int)(pj[j] * 9.999f)) * SL + R * lane;
+            const float dh = (pj[2] + 0.1f) * 0.1111111f;
+            const int u0 = (xloc - y0) - cA - 1;
+
+            S5 saved = D;
+            uint32_t bw = 0;
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                /* score: log(sum_a q_a d_a w_a) - shift + SS term */
+                float acc = q[k][0] * dw[0];
+#pragma unroll
+                for (int j = 1; j < 20; j++) acc = fmaf(q[k][j], dw[j], acc);
+                float sc = c2_logf_clamped(acc) - a.shift;
+                if (a.ssswgt > 0.0f) {
+                    float ss = sQT[co[0] + k] * pj[0];
+                    ss = fmaf(sQT[co[1] + k], pj[1], ss);
+                    ss = fmaf(sQT[co[2] + k], pj[2], ss);
+                    ss = fmaf(qh[k], dh, ss);
+                    ss = (ss - 0.05f) * a.ssswgt;
+                    sc = sc + ss;
+                }
+                if ((uint32_t)(u0 - k) >= (uint32_t)cW) sc = -9.0f;   /* distant corners */
+                S5 dg = saved;
+                if (k > 0 && (fixbits & (1u << k))) {
+                    if (first && kf == k) dg = s5_zero();
+                }
+                const S5 up = (k == 0) ? U : H[k - 1];
+                saved = H[k];
+                const S5 n = dp_cell(dg, up, H[k], qt[k], dt, sc);
+                H[k] = n;
+                if (n.mm > best) { best = n.mm; bpos = (t << 3) | k; }
+                if (BTCK) bw |= dp_btck(n) << (2 * k);
+            }
+            if (BTCK) btw[(size_t)t * 64 + lane] = bw;
             if (wbot && lane == 63) {
                 const int64_t c = c_begin + t - 63;
                 if (c >= c_begin && c < c_end) {
@@ -298,6 +318,23 @@ __global__ void k_zero_u64(unsigned long long* p, int64_t n)
     if (i < n) p[i] = 0ull;
 }
 
+/* wts[p][k] = 1 / (0.5 (qbkg_k + bkg_k(p))), the per-pair weights of the
+ * score's 20-term sum (the same FP32 operations as every other user) */
+__global__ void k_weights(const float* qbkg, const float* bkg, int nprof, float* wts)
+{
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nprof) return;
+#pragma unroll
+    for (int k = 0; k < 20; k++)
+        wts[(size_t)p * 20 + k] = 1.0f / (0.5f * (qbkg[k] + bkg[(size_t)k * nprof + p]));
+}
+
+__global__ void k_zero_u32(uint32_t* p, int64_t n)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = 0u;
+}
+
 }  // namespace
 
 /* clears the phase-1 maxima with a kernel (safe inside a captured graph) */
@@ -305,6 +342,23 @@ hipError_t c2_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t stre
 {
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, n);
+    return hipGetLastError();
+}
+
+hipError_t c2_launch_weights(const float* qbkg, const float* bkg, int nprof, float* wts, hipStream_t stream)
+{
+    if (nprof <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_weights, dim3((unsigned)((nprof + 255) / 256)), dim3(256), 0, stream, qbkg, bkg, nprof, wts);
+    return hipGetLastError();
+}
+
+/* clears n 32-bit words with a kernel (safe inside a captured graph) */
+hipError_t c2_launch_zero_u32(uint32_t* p, int64_t n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_zero_u32, dim3((unsigned)blocks), dim3(256), 0, stream, p, n);
     return hipGetLastError();
 }
 

@@ -41,6 +41,7 @@ int phase1_prepare(cbp_handle* h, QWork& w)
     const int np = h->dchunk.nprof;
     int rc;
     if ((rc = c2_dalloc(h, w.keys, sizeof(unsigned long long) * (size_t)np))) return rc;
+    if ((rc = c2_dalloc(h, w.wts, sizeof(float) * 20 * (size_t)np))) return rc;
     if (w.dq->hp.nstripes > 1 &&
         (rc = c2_dalloc(h, h->sbot[w.slot], sizeof(float) * 5 * (size_t)h->dchunk.npos))) return rc;
     if (w.hkeys_n < (size_t)np) {
@@ -65,7 +66,9 @@ int phase1_launch(cbp_handle* h, QWork& w, bool timed)
     if (dq->hp.nstripes > 1) a.bot = h->sbot[w.slot].as<float>();
     int nw = h->nw;
     while (nw > 1 && c2_phase1_lds_bytes(dq->hp.R, nw, dq->hp.nstripes) > 160 * 1024) nw >>= 1;
+    a.wts = w.wts.as<float>();
     PCHK(c2_launch_zero_u64(w.keys.as<unsigned long long>(), np, w.st));
+    PCHK(c2_launch_weights(dq->qbkg.as<float>(), h->dchunk.bkg.as<float>(), np, w.wts.as<float>(), w.st));
     if (timed) PCHK(hipEventRecord(w.e0, w.st));
     PCHK(c2_launch_phase1(a, dq->hp.R, nw, false, w.st));
     if (timed) PCHK(hipEventRecord(w.e1, w.st));
@@ -105,6 +108,7 @@ int stage_candidates(cbp_handle* h, QWork& w)
         a.bot = h->sbot[w.slot].as<float>();
     }
     a.seg_pb = w.dcand.as<int32_t>(); a.seg_pe = w.dpe.as<int32_t>(); a.nseg = nc;
+    a.wts = w.wts.as<float>();
     a.btck = w.btck.as<uint32_t>(); a.btck_off = w.doff.as<int64_t>();
     /* two waves per workgroup: the query tables in LDS are shared by two
      * candidates while two workgroups still fit a CU */
@@ -255,7 +259,9 @@ int stage_map(cbp_handle* h, QWork& w)
     if ((rc = c2_dalloc(h, w.mhand, sizeof(float) * (size_t)hoff))) return rc;
     if ((rc = c2_dalloc(h, w.mbt, (size_t)boff))) return rc;
     if ((rc = c2_dalloc(h, w.mstat, sizeof(float) * 4 * (size_t)nmap))) return rc;
-    if ((rc = c2_dalloc(h, w.malns, (size_t)nmap * 3 * w.alnstride))) return rc;
+    const size_t alnbytes = (size_t)nmap * 3 * w.alnstride;
+    const size_t alnwords = (alnbytes + 3) / 4;
+    if ((rc = c2_dalloc(h, w.malns, alnwords * 4))) return rc;
     if ((rc = c2_dalloc(h, w.malntmp, (size_t)nmap * 3 * w.alnstride))) return rc;
     if ((rc = c2_dalloc(h, w.mres, sizeof(int32_t) * 8 * (size_t)nmap))) return rc;
     ScoreCtx sc;
@@ -269,6 +275,8 @@ int stage_map(cbp_handle* h, QWork& w)
     ma.alns = w.malns.as<char>(); ma.alntmp = w.malntmp.as<char>(); ma.alnstride = w.alnstride;
     ma.res = w.mres.as<int32_t>(); ma.minpp = h->opt.minpp;
     ma.order = w.dmorder.as<int32_t>(); ma.R = mR; ma.ns = mns; ma.nw = mnw;
+    /* strings are NUL-padded to the stride: the whole buffer is output */
+    PCHK(c2_launch_zero_u32(w.malns.as<uint32_t>(), (int64_t)alnwords, w.st));
     PCHK(c2_launch_map(sc, ma, nmap, w.st));
     w.hmres.resize((size_t)8 * nmap);
     w.strs.resize((size_t)nmap * 3 * w.alnstride);

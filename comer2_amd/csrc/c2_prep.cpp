@@ -19,6 +19,7 @@
 #include <cstring>
 #include <algorithm>
 #include "c2_prep.h"
+#include "c2_kernels.h"
 #include "c2_fmath.h"
 
 namespace {
@@ -131,11 +132,17 @@ void c2_geometry(int Lq, int rmax, int* R, int* nstripes)
 {
     if (rmax < 1) rmax = 1;
     if (rmax > 5) rmax = 5;
-    int ns = (Lq + 64 * rmax - 1) / (64 * rmax);
-    if (ns < 1) ns = 1;
-    int r = (Lq + 64 * ns - 1) / (64 * ns);
-    if (r < 1) r = 1;
-    *R = r; *nstripes = ns;
+    for (;;) {
+        int ns = (Lq + 64 * rmax - 1) / (64 * rmax);
+        if (ns < 1) ns = 1;
+        int r = (Lq + 64 * ns - 1) / (64 * ns);
+        if (r < 1) r = 1;
+        /* a query of several stripes also stages its top boundary in the LDS
+         * ring: fewer rows per lane rather than fewer waves per workgroup */
+        if (ns > 1 && rmax > 1 && c2_phase1_lds_bytes(r, 8, ns) > 160 * 1024) { rmax--; continue; }
+        *R = r; *nstripes = ns;
+        return;
+    }
 }
 
 void c2_prepare_query(const cbp_profiles* q, int qp, const float* ssss, int rmax, QueryPrep* o)

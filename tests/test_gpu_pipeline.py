@@ -94,6 +94,7 @@ def test_batch_equals_one_by_one():
         assert np.array_equal(rb["alndata"].view(np.uint32), r1["alndata"].view(np.uint32))
         assert np.array_equal(rb["order"], r1["order"])
         assert rb["alns"] == r1["alns"]
+        assert np.array_equal(rb["aln_raw"], r1["aln_raw"])      # NUL-padded to the stride
     assert sum(r["nmap"] for r in batch) > 0
 
 
