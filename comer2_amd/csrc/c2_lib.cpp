@@ -642,3 +642,53 @@ extern "C" int cbp_autotune(cbp_handle* h, int32_t qidx, int32_t* seg_cols, int3
     if (nw) *nw = bn;
     return CBP_OK;
 }
+
+/* chunk limits for a query length (CuDeviceMemory::CalcMaxDbDataChunkSizeHelper,
+ * cupro/CuDeviceMemory.cu:334-... with GetTotalMemoryReqs,
+ * CuDeviceMemory.cuh:716-760): the most DB positions (and profiles, at the
+ * reference's expected profile length of 50) whose chunk data and phase-1
+ * buffers, plus the pass-2 buffers for 10% of those positions (the
+ * reference's DEV_PASS2MEMP default), fit the memory budget (or the device's
+ * free memory when the budget is 0).  This build never materialises a score
+ * matrix, so the per-position cost is the profile data itself, not
+ * Lq x 4 bytes of scores. */
+extern "C" int cbp_chunk_limits(cbp_handle* h, uint64_t qlen, uint64_t* maxpos, uint64_t* maxpros,
+                                uint64_t* maxpos2, uint64_t* maxpros2)
+{
+    if (!h || !maxpos || !maxpros || !maxpos2 || !maxpros2 || qlen < 1 || qlen > 65535) return CBP_ERR_ARG;
+    (void)hipSetDevice(h->device);
+    size_t avail = 0;
+    if (h->budget) {
+        avail = h->budget > h->used ? h->budget - h->used : 0;
+    } else {
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        avail = fr;
+    }
+    const double expct_len = 50.0, pass2 = 0.10;
+    int R = 1, ns = 1;
+    c2_geometry((int)qlen, h->rmax, &R, &ns);
+    int mR = 1, mns = 1, mnw = 1;
+    c2_map_geometry((int)qlen, &mR, &mns, &mnw);
+    /* phase 1, per DB position: 20 target frequencies, 3 SS probabilities,
+     * 7 transitions, residue, SS state, profile index; the stripe hand-off
+     * (5 floats) for multi-stripe queries; per profile: 20 background
+     * probabilities, ENO, length, offset, the extra transition row, the
+     * 64-bit maximum and the 20 score weights */
+    const double per_pos = 4.0 * (20 + 3 + 7) + 1 + 1 + 4 + (ns > 1 ? 20.0 : 0.0);
+    const double per_pro = 4.0 * (20 + 1 + 1 + 1 + 7 + 2 + 20);
+    const double p1 = per_pos + per_pro / expct_len;
+    /* pass 2, per DB position of a realigned pair: S', F (4 B each) and MAP
+     * backtrack (1 B) over 64*R*stripes rows; the candidates' 2-bit phase-1
+     * backtrack codes (4 B per step and lane per stripe); hand-off rows */
+    const double rows_m = 64.0 * mR * mns;
+    const double p2 = 9.0 * rows_m + 4.0 * 64.0 * ns + 4.0 * (5 * mns + 4);
+    double n = (double)avail / (p1 + pass2 * p2);
+    uint64_t mp = (uint64_t)n;
+    mp &= ~(uint64_t)31;                          /* positions padded to 32, as the reference */
+    *maxpos = mp;
+    *maxpros = (uint64_t)((double)mp / expct_len);
+    *maxpos2 = ((uint64_t)((double)mp * pass2)) & ~(uint64_t)31;
+    *maxpros2 = (uint64_t)((double)*maxpos2 / expct_len);
+    return CBP_OK;
+}

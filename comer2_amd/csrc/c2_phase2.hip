@@ -96,47 +96,71 @@ __global__ void k_finalize(ScoreCtx c, const int32_t* cand, int32_t ncand,
 
 constexpr int NSC = 641;
 
+/* query rows staged per tile: 20 target frequencies + p_H + 0.1 as 21
+ * words per row (broadcast reads), and the 30-entry SS table per row */
+constexpr int HTY = 256;
+constexpr int HQW = 21;
+
 __global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, int xuninf, int32_t* counts)
 {
     __shared__ int h[NSC];
+    __shared__ float sq[HTY * HQW];
+    __shared__ float sqt[30 * HTY];
+    __shared__ unsigned char sx[HTY];
     for (int i = threadIdx.x; i < NSC; i += blockDim.x) h[i] = 0;
-    __syncthreads();
     const int p = pro[blockIdx.x];
     const int Ldb = c.len[p];
     const int64_t col0 = c.dist[p];
-    for (int x = threadIdx.x; x < Ldb; x += blockDim.x) {
-        const int64_t col = col0 + x;
-        if (xuninf && c.aa[col] == 'X') continue;
-        float dw[20];
+    float w[20];
+    c2_pair_weights(c, p, w);
+    for (int x0 = 0; x0 < Ldb; x0 += blockDim.x) {
+        const int x = x0 + threadIdx.x;
+        const int64_t col = col0 + (x < Ldb ? x : 0);
+        const bool xok = x < Ldb && !(xuninf && c.aa[col] == 'X');
+        float dw[20], pj[3];
+        int co[3];
 #pragma unroll
-        for (int k = 0; k < 20; k++) {
-            const float w = 1.0f / (0.5f * (c.qbkg[k] + c.bkg[(size_t)k * c.nprof + p]));
-            dw[k] = c.trgfrq[(size_t)k * c.npos + col] * w;
-        }
-        float pj[3]; int co[3];
+        for (int k = 0; k < 20; k++) dw[k] = c.trgfrq[(size_t)k * c.npos + col] * w[k];
 #pragma unroll
         for (int j = 0; j < 3; j++) {
             pj[j] = c.ssprb[(size_t)j * c.npos + col];
             co[j] = 10 * j + (int)(pj[j] * 9.999f);
         }
         const float dh = (pj[2] + 0.1f) * 0.1111111f;
-        for (int y = 0; y < c.Lq; y++) {
-            if (xuninf && c.qaa[y] == 'X') continue;
-            float acc = c.qv[y] * dw[0];
-#pragma unroll
-            for (int k = 1; k < 20; k++) acc = fmaf(c.qv[(size_t)k * c.Lpad + y], dw[k], acc);
-            float sc = c2_logf_clamped(acc) - c.shift;
-            if (c.ssswgt > 0.0f) {
-                float ss = c.QT[(size_t)co[0] * c.Lpad + y] * pj[0];
-                ss = fmaf(c.QT[(size_t)co[1] * c.Lpad + y], pj[1], ss);
-                ss = fmaf(c.QT[(size_t)co[2] * c.Lpad + y], pj[2], ss);
-                ss = fmaf(c.qh[y], dh, ss);
-                ss = (ss - 0.05f) * c.ssswgt;
-                sc = sc + ss;
+        for (int y0 = 0; y0 < c.Lq; y0 += HTY) {
+            const int ny = min(HTY, c.Lq - y0);
+            __syncthreads();
+            for (int i = threadIdx.x; i < ny * HQW; i += blockDim.x) {
+                const int r = i / HQW, k = i - r * HQW;
+                sq[i] = (k < 20) ? c.qv[(size_t)k * c.Lpad + y0 + r] : c.qh[y0 + r];
             }
-            if (sc < -10.0f) sc = -10.0f; else if (sc > 10.0f) sc = 10.0f;
-            const int bin = (int)rintf(sc * 32.0f) + 320;
-            atomicAdd(&h[bin], 1);
+            for (int i = threadIdx.x; i < 30 * ny; i += blockDim.x) {
+                const int j = i / ny, r = i - j * ny;
+                sqt[j * HTY + r] = c.QT[(size_t)j * c.Lpad + y0 + r];
+            }
+            for (int i = threadIdx.x; i < ny; i += blockDim.x)
+                sx[i] = (xuninf && c.qaa[y0 + i] == 'X') ? 1 : 0;
+            __syncthreads();
+            if (!xok) continue;
+            for (int r = 0; r < ny; r++) {
+                if (sx[r]) continue;
+                const float* qr = sq + r * HQW;
+                float acc = qr[0] * dw[0];
+#pragma unroll
+                for (int k = 1; k < 20; k++) acc = fmaf(qr[k], dw[k], acc);
+                float sc = c2_logf_clamped(acc) - c.shift;
+                if (c.ssswgt > 0.0f) {
+                    float ss = sqt[co[0] * HTY + r] * pj[0];
+                    ss = fmaf(sqt[co[1] * HTY + r], pj[1], ss);
+                    ss = fmaf(sqt[co[2] * HTY + r], pj[2], ss);
+                    ss = fmaf(qr[20], dh, ss);
+                    ss = (ss - 0.05f) * c.ssswgt;
+                    sc = sc + ss;
+                }
+                if (sc < -10.0f) sc = -10.0f; else if (sc > 10.0f) sc = 10.0f;
+                const int bin = (int)rintf(sc * 32.0f) + 320;
+                atomicAdd(&h[bin], 1);
+            }
         }
     }
     __syncthreads();

@@ -51,6 +51,7 @@ def lib():
     L.cbp_last_kernel_ms.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.cbp_set_tuning.argtypes = [H, C.c_int32, C.c_int32, C.c_int32]
     L.cbp_set_graph.argtypes = [H, C.c_int32]
+    L.cbp_chunk_limits.argtypes = [H, C.c_uint64] + [C.POINTER(C.c_uint64)] * 4
     L.cbp_autotune.argtypes = [H, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.cbp_process.argtypes = [H, C.c_int32, C.c_float, C.c_float, C.POINTER(Results)]
     L.cbp_process_batch.argtypes = [H, C.c_void_p, C.c_int32, C.POINTER(Results)]
@@ -174,6 +175,25 @@ class CuBatchProcessing:
 
     def set_tuning(self, seg_cols=4096, rmax=5, nw=8):
         self._check(self.L.cbp_set_tuning(self.h, seg_cols, rmax, nw))
+
+    def chunk_limits(self, qlen):
+        """(max DB positions, max profiles, pass-2 positions, pass-2 profiles)
+        a chunk may hold for a query of qlen positions."""
+        v = [C.c_uint64() for _ in range(4)]
+        self._check(self.L.cbp_chunk_limits(self.h, qlen, *[C.byref(x) for x in v]))
+        return tuple(x.value for x in v)
+
+    def GetCurrentMaxDbPos(self, qlen):
+        return self.chunk_limits(qlen)[0]
+
+    def GetCurrentMaxNDbPros(self, qlen):
+        return self.chunk_limits(qlen)[1]
+
+    def GetCurrentMaxDbPosPass2(self, qlen):
+        return self.chunk_limits(qlen)[2]
+
+    def GetCurrentMaxNDbProsPass2(self, qlen):
+        return self.chunk_limits(qlen)[3]
 
     def set_graph(self, on=True):
         """Replay the phase-1 stage from a captured HIP graph while the batch,

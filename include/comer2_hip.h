@@ -160,6 +160,15 @@ int  cbp_score_block(cbp_handle* h, int32_t qidx, int32_t pronr, float* S, float
  * cudp/CuBatchDP_com.h:16-30) */
 int  cbp_set_tuning(cbp_handle* h, int32_t seg_cols, int32_t rmax, int32_t nw);
 
+/* chunk limits for a query of qlen positions: the most DB positions and
+ * profiles a chunk may hold, and the most positions and profiles of pass 2
+ * (statistics and MAP), under the handle's memory budget (device free memory
+ * when the budget is 0); replaces CuBatchProcessing::GetCurrentMaxDbPos,
+ * GetCurrentMaxNDbPros and their Pass2 variants (cupro/CuBatchProcessing.cuh:106-111)
+ * backed by CuDeviceMemory::CalcMaxDbDataChunkSize (cupro/CuDeviceMemory.cu:269-330) */
+int  cbp_chunk_limits(cbp_handle* h, uint64_t qlen, uint64_t* maxpos, uint64_t* maxpros,
+                      uint64_t* maxpos2, uint64_t* maxpros2);
+
 /* autotuner: times the phase-1 launch of query qidx over segment widths
  * {2048, 4096, 8192, 16384} columns and {4, 8} waves per workgroup, keeps the
  * fastest (cbp_set_tuning) and returns it; no reference counterpart */

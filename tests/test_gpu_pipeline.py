@@ -143,3 +143,20 @@ def test_autotune_keeps_results():
     assert sc in (2048, 4096, 8192, 16384) and nw in (4, 8)
     assert np.array_equal(cbp.phase1_keys(0), before)
     assert np.array_equal(cbp.phase1_keys(1), cbp.phase1_keys(1))
+
+
+def test_chunk_limits():
+    """Chunk limits shrink with the query length and grow with the budget;
+    pass 2 holds 10% of the positions at 50 positions per profile."""
+    from comer2_amd.host import CuBatchProcessing
+    small = CuBatchProcessing(0, mem_budget=8 << 30)
+    big = CuBatchProcessing(0, mem_budget=32 << 30)
+    a, b, c = small.chunk_limits(300), small.chunk_limits(2000), big.chunk_limits(300)
+    for lim in (a, b, c):
+        assert lim[0] % 32 == 0 and lim[2] % 32 == 0
+        assert lim[1] == lim[0] // 50 and lim[3] == lim[2] // 50
+        assert lim[2] <= lim[0] // 10 and lim[2] > lim[0] // 10 - 32
+    assert a[0] > b[0] > 0 and c[0] > 3 * a[0]
+    free = CuBatchProcessing(0).chunk_limits(300)
+    assert free[0] > 50_000_000          # the 50 M-position DB is one chunk on one GPU
+    small.close(); big.close()
