@@ -168,7 +168,7 @@ __global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, in
         counts[(size_t)blockIdx.x * NSC + i] = h[i];
 }
 
-/* ---------------- FP64 statistics (one thread per pair) ---------------- */
+/* ---------------- FP64 statistics (one wave per pair) ---------------- */
 
 constexpr int SMIN = -10, NUNS = 21, SCL = 32, SSMIN = -320;
 constexpr double LOG_SP_MIN = -8.7336545e+01, LOG_SP_MAX = 8.8722839e+01;
