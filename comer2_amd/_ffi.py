@@ -107,6 +107,8 @@ class SynthDB:
         return np.ctypeslib.as_array(self.view.dist, (self.nprof,)).copy()
 
     def __del__(self):
-        if getattr(self, "h", None) and SynthDB._lib is not None:
-            SynthDB._lib.c2s_free(self.h)
+        # at interpreter exit the class attribute may already be torn down
+        lib = getattr(type(self), "_lib", None) if type(self) is not None else None
+        if getattr(self, "h", None) and lib is not None:
+            lib.c2s_free(self.h)
             self.h = None
