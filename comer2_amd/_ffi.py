@@ -58,6 +58,8 @@ def load_synth():
                                  C.c_int32, C.c_int32, C.c_int32, C.c_double]
     lib.c2s_view.restype = C.POINTER(Profiles)
     lib.c2s_view.argtypes = [C.c_void_p]
+    lib.c2s_generate_lengths.restype = C.c_void_p
+    lib.c2s_generate_lengths.argtypes = [C.c_uint64, C.c_int32, C.c_void_p, C.c_double]
     lib.c2s_orgidx.restype = C.POINTER(C.c_int32)
     lib.c2s_orgidx.argtypes = [C.c_void_p]
     lib.c2s_free.argtypes = [C.c_void_p]
@@ -88,6 +90,15 @@ class SynthDB:
         n = SynthDB._lib.c2s_plant(self.h, C.byref(src.view), seed, nplant, kmin, kmax,
                                    ps.ctypes.data_as(C.c_void_p), pp.ctypes.data_as(C.c_void_p))
         return ps[:n], pp[:n]
+
+    @staticmethod
+    def from_lengths(seed, lengths, conc=20.0):
+        """Profiles of exactly these lengths, in this order."""
+        if SynthDB._lib is None:
+            SynthDB._lib = load_synth()
+        ln = np.ascontiguousarray(lengths, np.int32)
+        return SynthDB(0, 0, 0, _handle=SynthDB._lib.c2s_generate_lengths(
+            seed, ln.size, ln.ctypes.data_as(C.c_void_p), conc))
 
     def subset(self, p0, n):
         return SynthDB(0, 0, 0, _handle=SynthDB._lib.c2s_subset(self.h, p0, n))

@@ -130,10 +130,12 @@ extern "C" {
 /* lengths: LogNormal(log(median), sigma) rounded, clipped, or uniform on
  * [minlen, maxlen] when median <= 0; sorted descending when sort_desc != 0.
  * Returns an owned handle; free with c2s_free. */
+static void* generate_from(uint64_t seed, int32_t nprof, const std::vector<int32_t>& lens,
+                           int32_t sort_desc, double conc);
+
 void* c2s_generate(uint64_t seed, int32_t nprof, double median, double sigma,
                    int32_t minlen, int32_t maxlen, int32_t sort_desc, double conc)
 {
-    SynthDB* db = new SynthDB();
     std::vector<int32_t> lens(nprof);
     {
         Philox g(seed, 0xFFFFFFFFull);
@@ -146,6 +148,23 @@ void* c2s_generate(uint64_t seed, int32_t nprof, double median, double sigma,
             lens[p] = (int32_t)v;
         }
     }
+    return generate_from(seed, nprof, lens, sort_desc, conc);
+}
+
+/* profiles of the given lengths (in the given order), the same per-position
+ * distributions as c2s_generate */
+void* c2s_generate_lengths(uint64_t seed, int32_t nprof, const int32_t* lengths, double conc)
+{
+    std::vector<int32_t> lens(lengths, lengths + nprof);
+    return generate_from(seed, nprof, lens, 0, conc);
+}
+
+}  /* extern "C" */
+
+static void* generate_from(uint64_t seed, int32_t nprof, const std::vector<int32_t>& lens,
+                           int32_t sort_desc, double conc)
+{
+    SynthDB* db = new SynthDB();
     std::vector<int32_t> order(nprof);
     std::iota(order.begin(), order.end(), 0);
     if (sort_desc)
@@ -208,6 +227,8 @@ void* c2s_generate(uint64_t seed, int32_t nprof, double median, double sigma,
     v.aa = db->aa.data(); v.sss = db->sss.data(); v.trans = db->trans.data();
     return db;
 }
+
+extern "C" {
 
 const cbp_profiles* c2s_view(void* h) { return &((SynthDB*)h)->view; }
 const int32_t* c2s_orgidx(void* h) { return ((SynthDB*)h)->orgidx.data(); }

@@ -69,3 +69,35 @@ def test_map_matches_oracle(seed, cap, qlen):
             aligned += r.alnlen > 0
         total += len(sel)
     assert total > 0 and aligned > 0
+
+
+def test_map_edge_lengths():
+    """MAP realignment of every pair of edge-length query and profile lengths
+    (block and stripe edges, the corner-mask edge), down to 3 query positions
+    and 6 profile positions."""
+    EQ = [3, 31, 32, 33, 63, 64, 65, 97, 129]
+    ED = [6, 7, 31, 32, 33, 60, 61, 64, 65, 100, 177]
+    q = SynthDB.from_lengths(121, EQ)
+    d = SynthDB.from_lengths(122, ED)
+    orc = oracle()
+    cbp = processing(dict(max_map_pairs=0))
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    n = 0
+    for qp in range(q.nprof):
+        print(f"map edge: Lq={EQ[qp]}", flush=True)
+        res = cbp.ProcessScoreMatrix(qp, scorethld=-1.0, logevthld=50.0)
+        sel = [int(i) for i in res["order"] if res["logE"][i] <= 50.0]
+        assert list(res["map_rows"]) == sel
+        for k, row in enumerate(sel):
+            dp = int(res["pronr"][row])
+            S, M = orc.score_f32(q.view, qp, d.view, dp)
+            r, _, _, _, aln = orc.map(q.view, qp, d.view, dp, S, M, np.float32(res["epa"][row]))
+            ctx = (EQ[qp], ED[dp])
+            assert res["alnlen"][row] == r.alnlen, ctx
+            assert res["beg"][row] == ((r.begx << 16) | r.begy), ctx
+            assert res["end"][row] == ((r.x << 16) | r.y), ctx
+            assert res["alns"][k] == _strings(aln), ctx
+            n += 1
+    assert n > 0

@@ -25,3 +25,29 @@ def test_phase1_keys_bitexact(rmax, seg_cols, nw):
         bad = np.nonzero(got != exp)[0]
         assert bad.size == 0, "query %d: %d/%d pairs differ, first %s got %x exp %x" % (
             qp, bad.size, d.nprof, bad[:5], int(got[bad[0]]), int(exp[bad[0]]))
+
+
+EDGE_Q = [1, 2, 31, 32, 33, 63, 64, 65, 97, 319, 320, 321, 640, 641]
+EDGE_D = [1, 2, 3, 31, 32, 33, 60, 61, 64, 65, 100, 177, 1, 2, 3, 4, 5, 6, 7, 8]
+
+
+@pytest.mark.parametrize("rmax", [5, 2])
+def test_phase1_keys_edge_lengths(rmax):
+    """Query and profile lengths at the 32-row block edges, the lane-row and
+    stripe edges (64R rows), the corner-mask edge and down to 1, with runs of
+    very short profiles inside one segment."""
+    q = SynthDB.from_lengths(111, EDGE_Q)
+    d = SynthDB.from_lengths(112, EDGE_D)
+    orc = oracle()
+    cbp = processing()
+    cbp.set_tuning(128, rmax, 4)
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    for qp in range(q.nprof):
+        print(f"phase1 edge: rmax={rmax} Lq={EDGE_Q[qp]}", flush=True)
+        got = cbp.phase1_keys(qp)
+        exp = expected_keys(orc, q, qp, d)
+        bad = np.nonzero(got != exp)[0]
+        assert bad.size == 0, "Lq %d: profiles %s (lengths %s) got %s exp %s" % (
+            EDGE_Q[qp], bad[:5], [EDGE_D[i] for i in bad[:5]],
+            [hex(int(got[i])) for i in bad[:3]], [hex(int(exp[i])) for i in bad[:3]])

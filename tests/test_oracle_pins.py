@@ -324,3 +324,66 @@ def test_finalize_path_to_the_edge(orc):
     tot = sum(min(k - 1, 4) for k in range(1, Lq + 1))
     assert abs(f.score - (5.0 + math.sqrt(0.5 * 4.0 * tot))) <= 1e-5
     assert f.psts == Lq
+
+
+EDGE_Q = [1, 2, 31, 32, 33, 63, 64, 65, 97]
+EDGE_D = [1, 2, 3, 31, 32, 33, 60, 61, 64, 65, 100, 177]
+
+
+def test_viterbi_edge_sizes_match_reference_schedule(orc):
+    """Lengths at and around the reference's 32-row/-column block edges and the
+    corner-mask edge (min(60, rint(0.34 max(Lq, Ldb)))), down to 1: every cell
+    and backtrack byte matches the literal block-schedule emulation.  The
+    reference's running maximum is carried between blocks through a per-column
+    array whose write column can fall outside a short profile, so it can miss
+    cells the matrix holds; the oracle's maximum is over all cells, hence at
+    least the emulation's."""
+    q = SynthDB.from_lengths(101, EDGE_Q)
+    d = SynthDB.from_lengths(102, EDGE_D)
+    for qp in range(q.nprof):
+        for dp in range(d.nprof):
+            S, _ = orc.score_f32(q.view, qp, d.view, dp)
+            r1, b1, m1 = orc.viterbi_f32(q.view, qp, d.view, dp, S)
+            r2, b2, m2 = orc.refemu(q.view, qp, d.view, dp, S)
+            ctx = (EDGE_Q[qp], EDGE_D[dp])
+            assert np.array_equal(m1.view(np.uint32), m2.view(np.uint32)), ctx
+            assert np.array_equal(b1, b2), ctx
+            assert r1.maxsc >= r2.maxsc, ctx
+            assert r1.maxsc == max(0.0, float(m1.max())), ctx
+
+
+def test_viterbi_maximum_matches_reference_on_realistic_lengths(orc):
+    """On the benchmark's DB length distribution (log-normal, median 200,
+    clipped to 30-3000) the reference's carried maximum is the true one."""
+    d = SynthDB(2, 400, 200, 0.668, 30, 3000, sort_desc=False)
+    for qlen in (33, 300):
+        q = SynthDB(1002, 1, qlen, 0.0, qlen, qlen)
+        for dp in range(d.nprof):
+            S, _ = orc.score_f32(q.view, 0, d.view, dp)
+            r1, _, _ = orc.viterbi_f32(q.view, 0, d.view, dp, S)
+            r2, _, _ = orc.refemu(q.view, 0, d.view, dp, S)
+            assert (r1.maxsc, r1.x, r1.y) == (r2.maxsc, r2.x, r2.y), (qlen, dp)
+
+
+def test_map_edge_sizes_match_reference_schedule(orc):
+    """MAP at the same edge lengths: forward, forward+backward and backtrack
+    codes match the emulation everywhere; log Z (summed in the reference's
+    block order) matches for DB profiles of 6 positions or more, and the
+    realignment for queries of 3 positions or more."""
+    q = SynthDB.from_lengths(103, EDGE_Q)
+    d = SynthDB.from_lengths(104, EDGE_D)
+    for qp in range(q.nprof):
+        for dp in range(d.nprof):
+            S, M = orc.score_f32(q.view, qp, d.view, dp)
+            ctx = (EDGE_Q[qp], EDGE_D[dp])
+            r1, F1, FB1, b1, _ = orc.map(q.view, qp, d.view, dp, S, M, -19.0, direct=False)
+            r2, F2, FB2, b2, _ = orc.map(q.view, qp, d.view, dp, S, M, -19.0, direct=True)
+            assert np.array_equal(F1.view(np.uint32), F2.view(np.uint32)), ctx
+            assert np.array_equal(FB1.view(np.uint32), FB2.view(np.uint32)), ctx
+            assert np.array_equal(b1, b2), ctx
+            if EDGE_D[dp] >= 6:
+                assert r1.logZ == r2.logZ and r1.maxsc == r2.maxsc, ctx
+            else:
+                assert abs(r1.logZ - r2.logZ) <= 1e-6 * abs(r1.logZ), ctx
+            if EDGE_Q[qp] >= 3:
+                assert r1.alnlen == r2.alnlen, ctx
