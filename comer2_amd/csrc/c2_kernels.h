@@ -35,6 +35,18 @@ struct P1Args {
     const int64_t* btck_off;    /* per segment word offset */
 };
 
+/* device work queue of a batch's phase-1 sweeps: tile = (query slot, group
+ * of NW segments of that query's segment list) */
+struct P1Tile { int32_t q, g; };
+struct P1QueueArgs {
+    const P1Args* args;         /* [nq] per-query arguments (device) */
+    const P1Tile* tiles;        /* [ntiles] in pull order */
+    int32_t ntiles;
+    unsigned* head;             /* pull counter, zero before the launch */
+    float* botw;                /* per-wave stripe hand-off rows [grid*NW][5][bot_ld] (multi-stripe bins) */
+    int64_t bot_ld;             /* >= the columns of any segment in the queue */
+};
+
 /* everything needed to recompute one score-matrix cell */
 struct ScoreCtx {
     const float* trgfrq; const float* ssprb; const float* bkg; const int32_t* len;
@@ -111,6 +123,9 @@ hipError_t c2_launch_stats(const StatParams& sp, const float* nnw /*device [93]*
                            const int32_t* counts, const StatIn* in, int32_t n,
                            double* out /* [n][10] */, hipStream_t stream);
 hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream_t stream);
+/* persistent workgroups that fill the device once for k_phase1q<R,NW,btck> with lds bytes */
+int c2_phase1q_grid(int R, int NW, bool btck, size_t lds);
+hipError_t c2_launch_phase1q(const P1QueueArgs& q, int R, int NW, bool btck, int grid, size_t lds, hipStream_t stream);
 
 /* soft DP: one pair of (query, chunk profile) and its workspace offsets (floats) */
 struct SdpPair {

@@ -107,6 +107,7 @@ int cbp_destroy(cbp_handle* h)
     for (int s = 0; s < C2_NSTREAMS - 1; s++) if (h->gjoin[s]) (void)hipEventDestroy(h->gjoin[s]);
     if (h->gt0) (void)hipEventDestroy(h->gt0);
     if (h->gt1) (void)hipEventDestroy(h->gt1);
+    if (h->hqkeys) (void)hipHostFree(h->hqkeys);
     h->work.clear();
     h->qcache.clear();
     h->dchunk = DevSet();
@@ -180,6 +181,12 @@ int c2_build_segments(cbp_handle* h, const cbp_profiles* s)
     }
     if (h->seg_p.back() != s->nprof) h->seg_p.push_back(s->nprof);
     h->nseg = (int32_t)h->seg_p.size() - 1;
+    h->seg_maxcols = 0;
+    for (int32_t i = 0; i < h->nseg; i++) {
+        const int32_t pl = h->seg_p[i + 1] - 1;
+        const int64_t cols = (int64_t)s->dist[pl] + s->len[pl] - (int64_t)s->dist[h->seg_p[i]];
+        if (cols > h->seg_maxcols) h->seg_maxcols = (int32_t)cols;
+    }
     int rc;
     if ((rc = dupload(h, h->dseg_b, h->seg_p.data(), h->nseg))) return rc;
     if ((rc = dupload(h, h->dseg_e, h->seg_p.data() + 1, h->nseg))) return rc;

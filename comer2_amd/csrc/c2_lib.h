@@ -48,12 +48,17 @@ struct QWork {
     float scorethld = 0.0f, logevthld = 0.0f;
     hipStream_t st = nullptr;
     int slot = 0;                       /* stream slot (stripe hand-off buffer) */
-    DBuf keys;                          /* [nprof] phase-1 maxima */
-    unsigned long long* hkeys = nullptr; /* pinned host copy of keys */
+    DBuf keys;                          /* [nprof] phase-1 maxima (one-query runs) */
+    unsigned long long* hkeys = nullptr; /* pinned host copy of keys (one-query runs) */
     size_t hkeys_n = 0;
     std::vector<int32_t> cand, pe;
     std::vector<int64_t> off;
-    DBuf wts;                           /* [nprof][20] score weights of the query */
+    DBuf wts;                           /* [nprof][20] score weights of the query (one-query runs) */
+    /* the buffers the stages use: the query's own above, or its slice of a
+     * batch's work-queue buffers */
+    unsigned long long* dk = nullptr;   /* device maxima [nprof] */
+    unsigned long long* hk = nullptr;   /* pinned host copy [nprof] */
+    float* dw = nullptr;                /* device score weights [nprof][20] */
     DBuf dcand, dpe, doff, btck, fin;
     std::vector<FinRec> hfin;
     std::vector<int32_t> pass;
@@ -129,6 +134,22 @@ struct cbp_handle {
     struct GraphEnt { hipGraphExec_t exec = nullptr; int seen = 0; };
     bool use_graph = false;
     std::map<std::vector<int32_t>, GraphEnt> graphs;
+    /* device work queue of a batch (two queries or more): persistent
+     * phase-1 workgroups pull (query, segment group) tiles binned by rows per
+     * lane; the candidates' backtrack pass runs the same way */
+    struct QBin { int R = 1, NW = 8; bool tops = false; int64_t t0 = 0, nt = 0; int grid = 0; size_t lds = 0; int64_t bot0 = 0; double cost = 0.0; };
+    struct QPlan {
+        std::vector<P1Args> args;
+        std::vector<P1Tile> tiles;
+        std::vector<QBin> bins;
+        int64_t bot_ld = 0, bot_floats = 0;
+        DBuf dargs, dtiles, dhead, dbot;
+    };
+    bool use_queue = true;
+    QPlan qscan, qcand;
+    DBuf qkeys, qwts;                   /* [nq][nprof] maxima, [nq][nprof][20] weights */
+    unsigned long long* hqkeys = nullptr; size_t hqkeys_n = 0;
+    int32_t seg_maxcols = 0;            /* columns of the widest segment */
     hipEvent_t gfork = nullptr, gjoin[C2_NSTREAMS - 1] = {nullptr, nullptr, nullptr};
     hipEvent_t gt0 = nullptr, gt1 = nullptr;
     uint64_t chunk_gen = 0;             /* bumped by every chunk upload / tuning change */

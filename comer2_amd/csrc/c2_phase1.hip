@@ -44,15 +44,17 @@ constexpr int NF_BASE = 33, NF_TOP = 38;
  * hand-off buffer (stripes after the first) instead of the zero row -1 */
 /* the profile of the column lane loads in the refill of block t0 (loaded
  * half a block ahead, so the refill's loads do not wait on it) */
-__device__ __forceinline__ int block_profile(const P1Args& a, int lane, int t0, int64_t c_begin, int64_t c_end)
+template <class PA>
+__device__ __forceinline__ int block_profile(PA& a, int lane, int t0, int64_t c_begin, int64_t c_end)
 {
     const int64_t c = c_begin + t0 + lane;
     return (lane < BLK && c < c_end) ? a.addrpro[c] : -1;
 }
 
-template <int R>
-__device__ __forceinline__ void load_block(const P1Args& a, float* ring, int lane, int t0,
-                                           int64_t c_begin, int64_t c_end, bool tops, bool rtop, int p)
+template <int R, class PA>
+__device__ __forceinline__ void load_block(PA& a, float* ring, int lane, int t0,
+                                           int64_t c_begin, int64_t c_end, bool tops, bool rtop, int p,
+                                           const float* bot, int64_t bld, int64_t bc0)
 {
     if (lane >= BLK) return;
     const int slot = (t0 + lane) % RING;
@@ -94,9 +96,9 @@ __device__ __forceinline__ void load_block(const P1Args& a, float* ring, int lan
         ring[F_LEN * RING + slot] = __int_as_float(a.len[p]);
         ring[F_PID * RING + slot] = __int_as_float(p);
         if (tops) {
-            const float* bt = a.bot + c;
+            const float* bt = bot + (c - bc0);
             for (int s = 0; s < 5; s++)
-                ring[(F_TOP + s) * RING + slot] = rtop ? bt[(size_t)s * npos] : 0.0f;
+                ring[(F_TOP + s) * RING + slot] = rtop ? bt[(size_t)s * bld] : 0.0f;
         }
     } else {
         for (int k = 0; k < 20; k++) ring[(F_DW + k) * RING + slot] = 0.0f;
@@ -116,10 +118,18 @@ __device__ __forceinline__ unsigned long long mkkey(float best, int x, int y)
     return ((unsigned long long)__float_as_uint(best) << 32) | lo;
 }
 
-template <int R, int NW, bool BTCK>
-__global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
+/* P1Args read through the constant address space (loads may be scalar) */
+typedef const __attribute__((address_space(4))) P1Args CP1Args;
+
+/* the sweep of one segment (all stripes of the query) by wave wv of the
+ * workgroup; every wave of the workgroup calls it with the same query, and
+ * waves without a segment (seg >= nseg) still take part in the barriers.
+ * botw != nullptr: the stripe hand-off goes to the wave's own scratch rows
+ * of bot_ld columns (indexed from the segment's first column) instead of the
+ * query's per-column buffer a.bot */
+template <int R, int NW, bool BTCK, class PA>
+__device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw, int64_t bot_ld)
 {
-    extern __shared__ float lds[];
     const int SL = 64 * R;                      /* rows per stripe */
     float* sQT = lds;                           /* [30][SL] */
     float* sLL = sQT + 30 * SL;                 /* [5][SL] */
@@ -128,7 +138,6 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
     const int nf = tops ? NF_TOP : NF_BASE;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float* ring = sLD + 5 * SL + (size_t)wv * nf * RING;
-    const int seg = blockIdx.x * NW + wv;
 
     for (int s = 0; s < a.nstripes; s++) {
         const int rbase = s * SL;
@@ -148,6 +157,9 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
         const int ncols = (int)(c_end - c_begin);
         const int nsteps = ncols + 63;
         const int y0 = rbase + R * lane;
+        float* bot = a.bot;
+        int64_t bld = a.npos, bc0 = 0;
+        if (botw) { bot = botw; bld = bot_ld; bc0 = c_begin; }
 
         float q[R][20], qt[R][7], qh[R];
 #pragma unroll
@@ -175,7 +187,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
         }
         wave_lds_sync();
         int pn = block_profile(a, lane, 0, c_begin, c_end);
-        load_block<R>(a, ring, lane, 0, c_begin, c_end, tops, s > 0, pn);
+        load_block<R>(a, ring, lane, 0, c_begin, c_end, tops, s > 0, pn, bot, bld, bc0);
         wave_lds_sync();
 
         S5 H[R];
@@ -196,7 +208,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
         for (int t = 0; t < nsteps; t++) {
             if (((t + 1) % BLK) == 0) {
                 load_block<R>(a, ring, lane, t + 1, c_begin, c_end, tops, s > 0,
-                              block_profile(a, lane, t + 1, c_begin, c_end));
+                              block_profile(a, lane, t + 1, c_begin, c_end), bot, bld, bc0);
                 wave_lds_sync();
             }
 
@@ -284,9 +296,10 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
             if (wbot && lane == 63) {
                 const int64_t c = c_begin + t - 63;
                 if (c >= c_begin && c < c_end) {
-                    a.bot[0 * (size_t)a.npos + c] = H[R - 1].mm; a.bot[1 * (size_t)a.npos + c] = H[R - 1].mi;
-                    a.bot[2 * (size_t)a.npos + c] = H[R - 1].im; a.bot[3 * (size_t)a.npos + c] = H[R - 1].dn;
-                    a.bot[4 * (size_t)a.npos + c] = H[R - 1].nd;
+                    float* bt = bot + (c - bc0);
+                    bt[0 * (size_t)bld] = H[R - 1].mm; bt[1 * (size_t)bld] = H[R - 1].mi;
+                    bt[2 * (size_t)bld] = H[R - 1].im; bt[3 * (size_t)bld] = H[R - 1].dn;
+                    bt[4 * (size_t)bld] = H[R - 1].nd;
                 }
             }
             /* hand the last row to the next lane; lane 0 takes the stripe's top boundary */
@@ -309,6 +322,41 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
             atomicMax(a.keys + curpid, mkkey(best, (tb - lane) - pstart, y0 + kb));
         }
         if (tops) __threadfence();
+    }
+}
+
+template <int R, int NW, bool BTCK>
+__global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
+{
+    extern __shared__ float lds[];
+    p1_sweep<R, NW, BTCK, const P1Args>(a, blockIdx.x * NW + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)),
+                                        lds, nullptr, 0);
+}
+
+/* persistent form for a batch of queries: workgroups pull tiles (query,
+ * group of NW segments) from a device work queue until it is empty.  The
+ * tiles of one launch share the rows per lane R (one length bin) and come in
+ * the order the host sorted them (largest first); every wave hands its
+ * stripes over through its own scratch rows (5 x bot_ld floats per wave). */
+template <int R, int NW, bool BTCK>
+__global__ __launch_bounds__(64 * NW, 1) void k_phase1q(P1QueueArgs qa)
+{
+    extern __shared__ float lds[];
+    __shared__ int tile_s;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    float* botw = qa.botw ? qa.botw + ((size_t)blockIdx.x * NW + wv) * 5 * qa.bot_ld : nullptr;
+    for (;;) {
+        if (threadIdx.x == 0) tile_s = (int)atomicAdd(qa.head, 1u);
+        __syncthreads();
+        const int ti = __builtin_amdgcn_readfirstlane(tile_s);   /* uniform: the tile's arguments load as scalars */
+        __syncthreads();
+        if (ti >= qa.ntiles) break;
+        const int tq = __builtin_amdgcn_readfirstlane(qa.tiles[ti].q);
+        const int tg = __builtin_amdgcn_readfirstlane(qa.tiles[ti].g);
+        /* the arguments are read-only for the whole launch: read through the
+         * constant address space, as kernel arguments are */
+        CP1Args* ap = (CP1Args*)(qa.args + tq);
+        p1_sweep<R, NW, BTCK, CP1Args>(*ap, tg * NW + wv, lds, botw, qa.bot_ld);
     }
 }
 
@@ -417,4 +465,50 @@ hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream
         case 1: return launch_nw<1, false>(a, R, grid, lds, stream);
         default: return hipErrorInvalidValue;
     }
+}
+
+static constexpr size_t Q_MAX_DYN_LDS = 160 * 1024 - 256;
+
+/* the queue kernels built: the DB sweep with 8 or 4 waves per workgroup, the
+ * candidates' backtrack pass with 2 */
+static const void* phase1q_fn(int R, int NW, bool btck)
+{
+#define C2_P1Q(RR)                                                                   \
+    if (R == RR) {                                                                   \
+        if (btck) return NW == 2 ? (const void*)k_phase1q<RR, 2, true> : nullptr;    \
+        if (NW == 8) return (const void*)k_phase1q<RR, 8, false>;                    \
+        if (NW == 4) return (const void*)k_phase1q<RR, 4, false>;                    \
+        return nullptr;                                                              \
+    }
+    C2_P1Q(1) C2_P1Q(2) C2_P1Q(3) C2_P1Q(4) C2_P1Q(5)
+#undef C2_P1Q
+    return nullptr;
+}
+
+int c2_phase1q_grid(int R, int NW, bool btck, size_t lds)
+{
+    const void* fn = phase1q_fn(R, NW, btck);
+    if (!fn || lds > Q_MAX_DYN_LDS) return 0;
+    /* the kernel's own pull slot is static LDS: the dynamic limit stays below 160 KiB */
+    if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)Q_MAX_DYN_LDS) != hipSuccess) return 0;
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, fn, 64 * NW, lds) != hipSuccess || per < 1) {
+        /* the LDS bound (160 KiB per CU): extra workgroups only find the queue empty */
+        per = (int)((160 * 1024) / (lds + 256));
+        if (per < 1) per = 1;
+    }
+    return per * ncu;
+}
+
+hipError_t c2_launch_phase1q(const P1QueueArgs& q, int R, int NW, bool btck, int grid, size_t lds, hipStream_t stream)
+{
+    if (q.ntiles <= 0) return hipSuccess;
+    const void* fn = phase1q_fn(R, NW, btck);
+    if (!fn || grid < 1) return hipErrorInvalidValue;
+    P1QueueArgs qa = q;
+    void* args[] = {&qa};
+    return hipLaunchKernel(fn, dim3((unsigned)grid), dim3(64 * NW), args, lds, stream);
 }

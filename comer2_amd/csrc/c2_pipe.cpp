@@ -15,6 +15,13 @@
  * The queries of a batch are dealt to C2_NSTREAMS streams; the host work of
  * a stage (compaction, records, MAP selection) runs while no kernel is in
  * flight, so every query's results are those of a one-query run.
+ *
+ * A batch of two queries or more runs its phase-1 sweeps and the
+ * candidates' backtrack pass from a device work queue: the tiles (query,
+ * group of segments) of all queries are binned by rows per lane R (a kernel
+ * template parameter, set by the query length), sorted largest first inside
+ * a bin, and pulled by persistent workgroups, one launch per bin, the bins
+ * on separate streams so that one bin's tail overlaps the next bin.
  */
 #include <cstring>
 #include <cmath>
@@ -50,6 +57,9 @@ int phase1_prepare(cbp_handle* h, QWork& w)
         w.hkeys_n = np;
     }
     if (!w.e0) { PCHK(hipEventCreate(&w.e0)); PCHK(hipEventCreate(&w.e1)); }
+    w.dk = w.keys.as<unsigned long long>();
+    w.hk = w.hkeys;
+    w.dw = w.wts.as<float>();
     return CBP_OK;
 }
 
@@ -62,33 +72,215 @@ int phase1_launch(cbp_handle* h, QWork& w, bool timed)
     int rc;
     P1Args a;
     if ((rc = c2_fill_p1args(h, dq, &a))) return rc;
-    a.keys = w.keys.as<unsigned long long>();
+    a.keys = w.dk;
     if (dq->hp.nstripes > 1) a.bot = h->sbot[w.slot].as<float>();
     int nw = h->nw;
     while (nw > 1 && c2_phase1_lds_bytes(dq->hp.R, nw, dq->hp.nstripes) > 160 * 1024) nw >>= 1;
-    a.wts = w.wts.as<float>();
-    PCHK(c2_launch_zero_u64(w.keys.as<unsigned long long>(), np, w.st));
-    PCHK(c2_launch_weights(dq->qbkg.as<float>(), h->dchunk.bkg.as<float>(), np, w.wts.as<float>(), w.st));
+    a.wts = w.dw;
+    PCHK(c2_launch_zero_u64(w.dk, np, w.st));
+    PCHK(c2_launch_weights(dq->qbkg.as<float>(), h->dchunk.bkg.as<float>(), np, w.dw, w.st));
     if (timed) PCHK(hipEventRecord(w.e0, w.st));
     PCHK(c2_launch_phase1(a, dq->hp.R, nw, false, w.st));
     if (timed) PCHK(hipEventRecord(w.e1, w.st));
-    PCHK(hipMemcpyAsync(w.hkeys, w.keys.p, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, w.st));
+    PCHK(hipMemcpyAsync(w.hk, w.dk, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, w.st));
     return CBP_OK;
 }
 
-/* 2. candidates: backtrack codes recomputed, backtrace + correlation */
-int stage_candidates(cbp_handle* h, QWork& w)
+int ensure_fork_events(cbp_handle* h)
+{
+    if (!h->gfork) {
+        PCHK(hipEventCreateWithFlags(&h->gfork, hipEventDisableTiming));
+        for (int s = 0; s < C2_NSTREAMS - 1; s++) PCHK(hipEventCreateWithFlags(&h->gjoin[s], hipEventDisableTiming));
+    }
+    return CBP_OK;
+}
+
+/* uploads a queue plan (arguments, tiles, pull counters, per-wave hand-off
+ * rows) on the handle's stream; waits for the copies */
+int queue_upload(cbp_handle* h, cbp_handle::QPlan& P)
+{
+    int rc;
+    if ((rc = c2_dupload(h, P.dargs, P.args.data(), P.args.size(), h->stream))) return rc;
+    if ((rc = c2_dupload(h, P.dtiles, P.tiles.data(), P.tiles.size(), h->stream))) return rc;
+    if ((rc = c2_dalloc(h, P.dhead, sizeof(unsigned) * (P.bins.size() + 1)))) return rc;
+    if (P.bot_floats > 0 && (rc = c2_dalloc(h, P.dbot, sizeof(float) * (size_t)P.bot_floats))) return rc;
+    PCHK(hipStreamSynchronize(h->stream));
+    return CBP_OK;
+}
+
+/* bins of a queue plan: (R, multi-stripe) per query, the kernel's waves per
+ * workgroup, the persistent grid, and the bin's per-wave hand-off rows;
+ * tiles sorted by cost (stripes x columns) inside each bin, largest first */
+struct QTileCost { double cost; int32_t q, g; };
+int queue_bins(cbp_handle* h, cbp_handle::QPlan& P, int nq, bool btck,
+               const std::vector<std::vector<QTileCost>>& perq, const std::vector<int>& nwq)
+{
+    P.bins.clear(); P.tiles.clear();
+    std::map<std::pair<int, int>, int> key;
+    std::vector<std::vector<QTileCost>> bt;
+    for (int i = 0; i < nq; i++) {
+        if (perq[i].empty()) continue;
+        const QueryPrep& hp = h->work[i]->dq->hp;
+        const std::pair<int, int> k(hp.R, hp.nstripes > 1 ? 1 : 0);
+        auto it = key.find(k);
+        int b;
+        if (it == key.end()) {
+            b = (int)P.bins.size(); key[k] = b;
+            cbp_handle::QBin nb; nb.R = hp.R; nb.tops = hp.nstripes > 1; nb.NW = nwq[i];
+            P.bins.push_back(nb); bt.emplace_back();
+        } else b = it->second;
+        if (P.bins[b].NW != nwq[i]) return c2_fail(h, CBP_ERR_STATE, "queue bin with mixed waves per workgroup");
+        for (const QTileCost& t : perq[i]) { bt[b].push_back(t); P.bins[b].cost += t.cost; }
+    }
+    /* bins largest first: the first launched bin fills the device, later bins fill its tail */
+    std::vector<int> bo(P.bins.size());
+    for (size_t b = 0; b < bo.size(); b++) bo[b] = (int)b;
+    std::stable_sort(bo.begin(), bo.end(), [&](int x, int y) { return P.bins[x].cost > P.bins[y].cost; });
+    std::vector<cbp_handle::QBin> bins;
+    P.bot_floats = 0;
+    for (int b : bo) {
+        cbp_handle::QBin nb = P.bins[b];
+        std::vector<QTileCost>& v = bt[b];
+        std::stable_sort(v.begin(), v.end(), [](const QTileCost& x, const QTileCost& y) { return x.cost > y.cost; });
+        nb.t0 = (int64_t)P.tiles.size();
+        nb.nt = (int64_t)v.size();
+        for (const QTileCost& t : v) P.tiles.push_back(P1Tile{t.q, t.g});
+        nb.lds = c2_phase1_lds_bytes(nb.R, nb.NW, nb.tops ? 2 : 1);
+        if (nb.lds > 160 * 1024) return c2_fail(h, CBP_ERR_STATE, "queue bin exceeds the LDS");
+        nb.grid = c2_phase1q_grid(nb.R, nb.NW, btck, nb.lds);
+        if (nb.grid < 1) return c2_fail(h, CBP_ERR_HIP, "queue kernel occupancy");
+        nb.bot0 = P.bot_floats;
+        if (nb.tops) P.bot_floats += (int64_t)nb.grid * nb.NW * 5 * P.bot_ld;
+        bins.push_back(nb);
+    }
+    P.bins.swap(bins);
+    return CBP_OK;
+}
+
+/* launches the bins of an uploaded plan: pull counters cleared, the
+ * handle's stream forks to one stream per bin (up to C2_NSTREAMS) and joins
+ * them back (capturable) */
+int queue_launch(cbp_handle* h, cbp_handle::QPlan& P, bool btck)
+{
+    const int nb = (int)P.bins.size();
+    if (nb == 0) return CBP_OK;
+    int rc;
+    if ((rc = ensure_fork_events(h))) return rc;
+    PCHK(c2_launch_zero_u32(P.dhead.as<uint32_t>(), nb, h->stream));
+    const int ns = nb < C2_NSTREAMS ? nb : C2_NSTREAMS;
+    PCHK(hipEventRecord(h->gfork, h->stream));
+    for (int s = 1; s < ns; s++) PCHK(hipStreamWaitEvent(slot_stream(h, s), h->gfork, 0));
+    for (int b = 0; b < nb; b++) {
+        const cbp_handle::QBin& B = P.bins[b];
+        P1QueueArgs qa;
+        qa.args = P.dargs.as<P1Args>();
+        qa.tiles = P.dtiles.as<P1Tile>() + B.t0;
+        qa.ntiles = (int32_t)B.nt;
+        qa.head = P.dhead.as<unsigned>() + b;
+        qa.botw = B.tops ? P.dbot.as<float>() + B.bot0 : nullptr;
+        qa.bot_ld = P.bot_ld;
+        PCHK(c2_launch_phase1q(qa, B.R, B.NW, btck, B.grid, B.lds, slot_stream(h, b % ns)));
+    }
+    for (int s = 1; s < ns; s++) {
+        PCHK(hipEventRecord(h->gjoin[s - 1], slot_stream(h, s)));
+        PCHK(hipStreamWaitEvent(h->stream, h->gjoin[s - 1], 0));
+    }
+    return CBP_OK;
+}
+
+/* 1a (batch). the work-queue plan of the batch's DB sweeps: maxima, pinned
+ * copies and weights as slices of batch buffers, per-query arguments, tiles
+ * of NW segments */
+int queue_scan_prepare(cbp_handle* h, int nq)
+{
+    const int np = h->dchunk.nprof;
+    int rc;
+    if ((rc = c2_dalloc(h, h->qkeys, sizeof(unsigned long long) * (size_t)nq * np))) return rc;
+    if ((rc = c2_dalloc(h, h->qwts, sizeof(float) * 20 * (size_t)nq * np))) return rc;
+    if (h->hqkeys_n < (size_t)nq * np) {
+        if (h->hqkeys) { (void)hipHostFree(h->hqkeys); h->hqkeys = nullptr; h->hqkeys_n = 0; }
+        PCHK(hipHostMalloc((void**)&h->hqkeys, sizeof(unsigned long long) * (size_t)nq * np, hipHostMallocDefault));
+        h->hqkeys_n = (size_t)nq * np;
+    }
+    cbp_handle::QPlan& P = h->qscan;
+    P.args.resize(nq);
+    P.bot_ld = h->seg_maxcols;
+    const cbp_profiles& ch = h->chunk.v;
+    std::vector<std::vector<QTileCost>> perq(nq);
+    std::vector<int> nwq(nq);
+    for (int i = 0; i < nq; i++) {
+        QWork& w = *h->work[i];
+        if (!w.e0) { PCHK(hipEventCreate(&w.e0)); PCHK(hipEventCreate(&w.e1)); }
+        w.dk = h->qkeys.as<unsigned long long>() + (size_t)i * np;
+        w.hk = h->hqkeys + (size_t)i * np;
+        w.dw = h->qwts.as<float>() + (size_t)i * np * 20;
+        const QueryPrep& hp = w.dq->hp;
+        if ((rc = c2_fill_p1args(h, w.dq, &P.args[i]))) return rc;
+        P.args[i].keys = w.dk;
+        P.args[i].wts = w.dw;
+        int nw = h->nw >= 8 ? 8 : 4;
+        if (nw == 8 && c2_phase1_lds_bytes(hp.R, 8, hp.nstripes) > 160 * 1024) nw = 4;
+        nwq[i] = nw;
+        const int ng = (h->nseg + nw - 1) / nw;
+        perq[i].resize(ng);
+        for (int g = 0; g < ng; g++) {
+            const int s0 = g * nw, s1 = std::min(h->nseg, s0 + nw);
+            const int32_t pb = h->seg_p[s0], pl = h->seg_p[s1] - 1;
+            const double cols = (double)((int64_t)ch.dist[pl] + ch.len[pl] - (int64_t)ch.dist[pb]) / (s1 - s0);
+            perq[i][g] = QTileCost{(double)hp.nstripes * hp.R * (cols + 63.0), i, g};
+        }
+    }
+    if ((rc = queue_bins(h, P, nq, false, perq, nwq))) return rc;
+    return queue_upload(h, P);
+}
+
+/* 1b (batch). maxima of every (query, DB profile) pair of the batch
+ * (capturable) */
+int queue_scan_launch(cbp_handle* h, int nq, bool timed)
+{
+    const int np = h->dchunk.nprof;
+    int rc;
+    PCHK(c2_launch_zero_u64(h->qkeys.as<unsigned long long>(), (int64_t)nq * np, h->stream));
+    for (int i = 0; i < nq; i++) {
+        QWork& w = *h->work[i];
+        if (w.dq->hp.R <= 4)
+            PCHK(c2_launch_weights(w.dq->qbkg.as<float>(), h->dchunk.bkg.as<float>(), np, w.dw, h->stream));
+    }
+    if (timed) PCHK(hipEventRecord(h->ev0, h->stream));
+    if ((rc = queue_launch(h, h->qscan, false))) return rc;
+    if (timed) PCHK(hipEventRecord(h->ev1, h->stream));
+    PCHK(hipMemcpyAsync(h->hqkeys, h->qkeys.p, sizeof(unsigned long long) * (size_t)nq * np,
+                        hipMemcpyDeviceToHost, h->stream));
+    return CBP_OK;
+}
+
+/* backtrack-pass arguments of a query: its candidate profiles as segments */
+int cand_args(cbp_handle* h, QWork& w, P1Args* a)
+{
+    (void)h;
+    a->keys = w.dk;
+    a->wts = w.dw;
+    a->seg_pb = w.dcand.as<int32_t>(); a->seg_pe = w.dpe.as<int32_t>(); a->nseg = (int32_t)w.cand.size();
+    a->btck = w.btck.as<uint32_t>(); a->btck_off = w.doff.as<int64_t>();
+    return CBP_OK;
+}
+
+/* 2a. candidates (maximum above the score threshold, in profile order), their
+ * backtrack-code layout and the arguments of the backtrack pass (one
+ * segment per candidate profile); launches the pass unless the batch's work
+ * queue runs it */
+int cand_prepare(cbp_handle* h, QWork& w, bool launch)
 {
     DevQuery* dq = w.dq;
     const cbp_profiles& ch = h->chunk.v;
     const int np = h->dchunk.nprof;
     w.cand.clear();
     for (int p = 0; p < np; p++)
-        if (w.hkeys[p] && u2f((uint32_t)(w.hkeys[p] >> 32)) > w.scorethld) w.cand.push_back(p);
+        if (w.hk[p] && u2f((uint32_t)(w.hk[p] >> 32)) > w.scorethld) w.cand.push_back(p);
     const int nc = (int)w.cand.size();
     w.hfin.resize(nc);
     if (nc == 0) return CBP_OK;
-    const int R = dq->hp.R, ns = dq->hp.nstripes;
+    const int ns = dq->hp.nstripes;
     w.off.assign(nc + 1, 0);
     w.pe.resize(nc);
     for (int i = 0; i < nc; i++) {
@@ -100,27 +292,72 @@ int stage_candidates(cbp_handle* h, QWork& w)
     if ((rc = c2_dupload(h, w.doff, w.off.data(), nc + 1, w.st))) return rc;
     if ((rc = c2_dupload(h, w.dpe, w.pe.data(), nc, w.st))) return rc;
     if ((rc = c2_dalloc(h, w.btck, sizeof(uint32_t) * (size_t)w.off[nc]))) return rc;
+    if ((rc = c2_dalloc(h, w.fin, sizeof(FinRec) * nc))) return rc;
+    if (!launch) return CBP_OK;
     P1Args a;
     if ((rc = c2_fill_p1args(h, dq, &a))) return rc;
-    a.keys = w.keys.as<unsigned long long>();
+    if ((rc = cand_args(h, w, &a))) return rc;
     if (ns > 1) {
         if ((rc = c2_dalloc(h, h->sbot[w.slot], sizeof(float) * 5 * (size_t)h->dchunk.npos))) return rc;
         a.bot = h->sbot[w.slot].as<float>();
     }
-    a.seg_pb = w.dcand.as<int32_t>(); a.seg_pe = w.dpe.as<int32_t>(); a.nseg = nc;
-    a.wts = w.wts.as<float>();
-    a.btck = w.btck.as<uint32_t>(); a.btck_off = w.doff.as<int64_t>();
     /* two waves per workgroup: the query tables in LDS are shared by two
      * candidates while two workgroups still fit a CU */
     int nwb = 2;
-    while (nwb > 1 && c2_phase1_lds_bytes(R, nwb, ns) > 80 * 1024) nwb >>= 1;
-    PCHK(c2_launch_phase1(a, R, nwb, true, w.st));
-    if ((rc = c2_dalloc(h, w.fin, sizeof(FinRec) * nc))) return rc;
+    while (nwb > 1 && c2_phase1_lds_bytes(dq->hp.R, nwb, ns) > 80 * 1024) nwb >>= 1;
+    PCHK(c2_launch_phase1(a, dq->hp.R, nwb, true, w.st));
+    return CBP_OK;
+}
+
+/* 2b. finalize (backtrace + correlation bonus) of the candidates and their
+ * records on the host */
+int cand_finalize(cbp_handle* h, QWork& w)
+{
+    const int nc = (int)w.cand.size();
+    if (nc == 0) return CBP_OK;
     ScoreCtx sc;
-    c2_fill_scorectx(h, dq, &sc);
-    PCHK(c2_launch_finalize(sc, w.dcand.as<int32_t>(), nc, w.keys.as<unsigned long long>(),
+    c2_fill_scorectx(h, w.dq, &sc);
+    PCHK(c2_launch_finalize(sc, w.dcand.as<int32_t>(), nc, w.dk,
                             w.btck.as<uint32_t>(), w.doff.as<int64_t>(), w.fin.as<FinRec>(), w.st));
     PCHK(hipMemcpyAsync(w.hfin.data(), w.fin.p, sizeof(FinRec) * nc, hipMemcpyDeviceToHost, w.st));
+    return CBP_OK;
+}
+
+/* 2 (batch). the candidates' backtrack pass of every query from the work
+ * queue: tiles of two candidate profiles, binned by R; waits for the pass */
+int queue_cand_run(cbp_handle* h, int nq)
+{
+    cbp_handle::QPlan& P = h->qcand;
+    const cbp_profiles& ch = h->chunk.v;
+    P.args.resize(nq);
+    std::vector<std::vector<QTileCost>> perq(nq);
+    std::vector<int> nwq(nq, 2);
+    int64_t maxlen = 1;
+    int rc;
+    for (int i = 0; i < nq; i++) {
+        QWork& w = *h->work[i];
+        const int nc = (int)w.cand.size();
+        if ((rc = c2_fill_p1args(h, w.dq, &P.args[i]))) return rc;
+        if (nc == 0) continue;
+        if ((rc = cand_args(h, w, &P.args[i]))) return rc;
+        const QueryPrep& hp = w.dq->hp;
+        const int ng = (nc + 1) / 2;
+        perq[i].resize(ng);
+        for (int g = 0; g < ng; g++) {
+            const int l0 = ch.len[w.cand[2 * g]], l1 = (2 * g + 1 < nc) ? ch.len[w.cand[2 * g + 1]] : 0;
+            perq[i][g] = QTileCost{(double)hp.nstripes * hp.R * (std::max(l0, l1) + 63.0), i, g};
+            if (l0 > maxlen) maxlen = l0;
+            if (l1 > maxlen) maxlen = l1;
+        }
+    }
+    P.bot_ld = maxlen;
+    if ((rc = queue_bins(h, P, nq, true, perq, nwq))) return rc;
+    if (P.bins.empty()) return CBP_OK;
+    /* the candidate lists were uploaded on the query streams */
+    for (int s = 0; s < C2_NSTREAMS && s < nq; s++) PCHK(hipStreamSynchronize(slot_stream(h, s)));
+    if ((rc = queue_upload(h, P))) return rc;
+    if ((rc = queue_launch(h, P, true))) return rc;
+    PCHK(hipStreamSynchronize(h->stream));
     return CBP_OK;
 }
 
@@ -326,20 +563,22 @@ int sync_slots(cbp_handle* h, int nslots)
 
 /* captures the phase-1 stage of the batch: the handle's stream forks to the
  * other slots' streams through an event and joins them back */
-int build_graph(cbp_handle* h, int nq, int nslots, hipGraphExec_t* out)
+int build_graph(cbp_handle* h, int nq, int nslots, bool queue, hipGraphExec_t* out)
 {
-    if (!h->gfork) {
-        PCHK(hipEventCreateWithFlags(&h->gfork, hipEventDisableTiming));
-        for (int s = 0; s < C2_NSTREAMS - 1; s++) PCHK(hipEventCreateWithFlags(&h->gjoin[s], hipEventDisableTiming));
-    }
+    int rc = ensure_fork_events(h);
+    if (rc) return rc;
     PCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
-    int rc = CBP_OK;
-    hipError_t e = hipEventRecord(h->gfork, h->stream);
-    for (int s = 1; e == hipSuccess && s < nslots; s++) e = hipStreamWaitEvent(slot_stream(h, s), h->gfork, 0);
-    for (int i = 0; e == hipSuccess && rc == CBP_OK && i < nq; i++) rc = phase1_launch(h, *h->work[i], false);
-    for (int s = 1; e == hipSuccess && s < nslots; s++) {
-        e = hipEventRecord(h->gjoin[s - 1], slot_stream(h, s));
-        if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->gjoin[s - 1], 0);
+    hipError_t e = hipSuccess;
+    if (queue) {
+        rc = queue_scan_launch(h, nq, false);
+    } else {
+        e = hipEventRecord(h->gfork, h->stream);
+        for (int s = 1; e == hipSuccess && s < nslots; s++) e = hipStreamWaitEvent(slot_stream(h, s), h->gfork, 0);
+        for (int i = 0; e == hipSuccess && rc == CBP_OK && i < nq; i++) rc = phase1_launch(h, *h->work[i], false);
+        for (int s = 1; e == hipSuccess && s < nslots; s++) {
+            e = hipEventRecord(h->gjoin[s - 1], slot_stream(h, s));
+            if (e == hipSuccess) e = hipStreamWaitEvent(h->stream, h->gjoin[s - 1], 0);
+        }
     }
     hipGraph_t g = nullptr;
     const hipError_t ee = hipStreamEndCapture(h->stream, &g);
@@ -387,7 +626,19 @@ int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* 
         if (!w.dq) return rc;
     }
     PCHK(hipStreamSynchronize(h->stream));
-    for (int i = 0; i < nq; i++) if ((rc = phase1_prepare(h, *h->work[i]))) return rc;
+    const bool queue = h->use_queue && nq >= 2;
+    if (queue) {
+        /* a captured graph holds the batch buffers' addresses: a buffer that
+         * grows retires every graph */
+        const void* old[7] = {h->qkeys.p, h->qwts.p, h->hqkeys, h->qscan.dargs.p, h->qscan.dtiles.p,
+                              h->qscan.dhead.p, h->qscan.dbot.p};
+        if ((rc = queue_scan_prepare(h, nq))) return rc;
+        const void* now[7] = {h->qkeys.p, h->qwts.p, h->hqkeys, h->qscan.dargs.p, h->qscan.dtiles.p,
+                              h->qscan.dhead.p, h->qscan.dbot.p};
+        if (std::memcmp(old, now, sizeof old) != 0) drop_graphs(h);
+    } else {
+        for (int i = 0; i < nq; i++) if ((rc = phase1_prepare(h, *h->work[i]))) return rc;
+    }
     bool graphed = false;
     if (h->use_graph) {
         /* a graph holds its batch's buffers (per-slot keys, pinned copies,
@@ -399,12 +650,12 @@ int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* 
             const std::vector<int32_t>& k0 = h->graphs.begin()->first;
             if (k0[0] != glo || k0[1] != ghi) drop_graphs(h);
         }
-        std::vector<int32_t> key = {glo, ghi, nq};
+        std::vector<int32_t> key = {glo, ghi, queue ? 1 : 0, nq};
         key.insert(key.end(), qidx, qidx + nq);
         if (!h->graphs.count(key) && h->graphs.size() >= 8) drop_graphs(h);
         cbp_handle::GraphEnt& ge = h->graphs[key];
         /* capture after one eager run of the batch (kernel attributes are set) */
-        if (!ge.exec && ge.seen >= 1 && (rc = build_graph(h, nq, nslots, &ge.exec))) return rc;
+        if (!ge.exec && ge.seen >= 1 && (rc = build_graph(h, nq, nslots, queue, &ge.exec))) return rc;
         if (ge.exec) {
             if (!h->gt0) { PCHK(hipEventCreate(&h->gt0)); PCHK(hipEventCreate(&h->gt1)); }
             PCHK(hipEventRecord(h->gt0, h->stream));
@@ -417,10 +668,17 @@ int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* 
         }
     }
     if (!graphed) {
-        for (int i = 0; i < nq; i++) if ((rc = phase1_launch(h, *h->work[i], true))) return rc;
-        if ((rc = sync_slots(h, nslots))) return rc;
+        if (queue) {
+            if ((rc = queue_scan_launch(h, nq, true))) return rc;
+            PCHK(hipStreamSynchronize(h->stream));
+        } else {
+            for (int i = 0; i < nq; i++) if ((rc = phase1_launch(h, *h->work[i], true))) return rc;
+            if ((rc = sync_slots(h, nslots))) return rc;
+        }
     }
-    for (int i = 0; i < nq; i++) if ((rc = stage_candidates(h, *h->work[i]))) return rc;
+    for (int i = 0; i < nq; i++) if ((rc = cand_prepare(h, *h->work[i], !queue))) return rc;
+    if (queue && (rc = queue_cand_run(h, nq))) return rc;
+    for (int i = 0; i < nq; i++) if ((rc = cand_finalize(h, *h->work[i]))) return rc;
     if ((rc = sync_slots(h, nslots))) return rc;
     for (int i = 0; i < nq; i++) if ((rc = stage_stats(h, *h->work[i]))) return rc;
     if ((rc = sync_slots(h, nslots))) return rc;
@@ -429,9 +687,10 @@ int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* 
     for (int i = 0; i < nq; i++) stage_results(h, *h->work[i], &outs[i]);
     /* device time of the phase-1 launches (the whole graph when replayed) */
     double ms = 0.0;
-    if (graphed) {
+    if (graphed || queue) {
         float t = 0.0f;
-        PCHK(hipEventElapsedTime(&t, h->gt0, h->gt1));
+        if (graphed) PCHK(hipEventElapsedTime(&t, h->gt0, h->gt1));
+        else PCHK(hipEventElapsedTime(&t, h->ev0, h->ev1));
         ms = t;
     } else {
         for (int i = 0; i < nq; i++) {
@@ -441,7 +700,7 @@ int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* 
         }
     }
     h->last_ms = ms;
-    h->last_nlaunch = nq;
+    h->last_nlaunch = (graphed || queue) ? 1 : nq;
     return CBP_OK;
 }
 
@@ -471,6 +730,14 @@ extern "C" int cbp_set_graph(cbp_handle* h, int32_t on)
 {
     if (!h) return CBP_ERR_ARG;
     h->use_graph = on != 0;
+    drop_graphs(h);
+    return CBP_OK;
+}
+
+extern "C" int cbp_set_queue(cbp_handle* h, int32_t on)
+{
+    if (!h) return CBP_ERR_ARG;
+    h->use_queue = on != 0;
     drop_graphs(h);
     return CBP_OK;
 }

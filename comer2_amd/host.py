@@ -51,6 +51,7 @@ def lib():
     L.cbp_last_kernel_ms.argtypes = [H, C.POINTER(C.c_double), C.POINTER(C.c_int32)]
     L.cbp_set_tuning.argtypes = [H, C.c_int32, C.c_int32, C.c_int32]
     L.cbp_set_graph.argtypes = [H, C.c_int32]
+    L.cbp_set_queue.argtypes = [H, C.c_int32]
     L.cbp_chunk_limits.argtypes = [H, C.c_uint64] + [C.POINTER(C.c_uint64)] * 4
     L.cbp_autotune.argtypes = [H, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.cbp_process.argtypes = [H, C.c_int32, C.c_float, C.c_float, C.POINTER(Results)]
@@ -199,6 +200,12 @@ class CuBatchProcessing:
         """Replay the phase-1 stage from a captured HIP graph while the batch,
         chunk and tuning stay unchanged."""
         self._check(self.L.cbp_set_graph(self.h, 1 if on else 0))
+
+    def set_queue(self, on=True):
+        """Batches of two queries or more run phase 1 and the candidates'
+        backtrack pass from the device work queue (on, the default) or one
+        launch per query (off)."""
+        self._check(self.L.cbp_set_queue(self.h, 1 if on else 0))
 
     def autotune(self, qidx=0):
         """Times phase 1 of query qidx over a small grid of segment widths and

@@ -179,6 +179,15 @@ int  cbp_autotune(cbp_handle* h, int32_t qidx, int32_t* seg_cols, int32_t* nw);
  * run of a batch is eager); no reference counterpart */
 int  cbp_set_graph(cbp_handle* h, int32_t on);
 
+/* phase-1 sweeps and the candidates' backtrack pass of a cbp_process_batch
+ * call of two queries or more from a device work queue (on = 1, the
+ * default): persistent workgroups pull (query, segment group) tiles binned
+ * by rows per lane, one launch per bin; on = 0 launches each query on its
+ * own (query streams).  Results are identical either way; no reference
+ * counterpart (the reference's DevCommThread runs one query at a time,
+ * cualn/DevCommThread.cpp:296-356) */
+int  cbp_set_queue(cbp_handle* h, int32_t on);
+
 /* device-side elapsed time of the last cbp_phase1_keys / cbp_process call's
  * dominant kernel (ms, HIP events on its stream) and the number of launches */
 int  cbp_last_kernel_ms(const cbp_handle* h, double* ms, int32_t* nlaunch);

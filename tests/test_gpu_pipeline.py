@@ -160,3 +160,39 @@ def test_chunk_limits():
     free = CuBatchProcessing(0).chunk_limits(300)
     assert free[0] > 50_000_000          # the 50 M-position DB is one chunk on one GPU
     small.close(); big.close()
+
+
+def _same(a, b):
+    return (a["npassed"] == b["npassed"] and a["nmap"] == b["nmap"]
+            and np.array_equal(a["alndata"].view(np.uint32), b["alndata"].view(np.uint32))
+            and np.array_equal(a["order"], b["order"]) and np.array_equal(a["aln_raw"], b["aln_raw"]))
+
+
+def test_work_queue_batch_equals_per_query_launches():
+    """A batch whose queries fall into every rows-per-lane bin (1..5), one
+    and several stripes, runs phase 1 and the backtrack pass from the device
+    work queue; the hit lists equal those of per-query launches and of
+    one-query runs bit for bit, also after the batch buffers grow."""
+    lens = [40, 100, 150, 230, 300, 350, 500, 700, 1000]
+    q = SynthDB.from_lengths(33, lens)
+    d = SynthDB(34, 160, 180, 0.668, 30, 900)
+    d.plant(q, 35, 40, 3.0, 60.0)
+    cbp = processing()
+    cbp.set_tuning(1024, 5, 8)           # many tiles per query
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    ql = q.lengths()
+    qs = [int(i) for i in np.argsort(-ql, kind="stable")][::2] + [int(i) for i in np.argsort(-ql, kind="stable")][1::2]
+    small = cbp.ProcessBatch(qs[:3])                 # smaller batch first: buffers grow later
+    queued = cbp.ProcessBatch(qs)
+    cbp.set_queue(False)
+    launched = cbp.ProcessBatch(qs)
+    cbp.set_queue(True)
+    for k, qp in enumerate(qs):
+        one = cbp.ProcessScoreMatrix(qp)
+        assert _same(queued[k], one), (k, qp, int(ql[qp]))
+        assert _same(launched[k], one), (k, qp, int(ql[qp]))
+        if k < 3:
+            assert _same(small[k], one), (k, qp)
+    assert sum(r["npassed"] for r in queued) > len(qs)
