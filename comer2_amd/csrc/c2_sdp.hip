@@ -78,6 +78,72 @@ __device__ __forceinline__ void ld_trn(const float* T, int64_t ntr, int64_t base
     t[6] = T[6 * ntr + base + i] * L2E;
 }
 
+/* Row-major S, dS and posterior blocks are read and written four columns at
+ * a time: at anti-diagonal step t lane l holds column t-l of each of its R
+ * rows, so the 4 steps of an aligned group (t = 4m..4m+3) cover 4
+ * consecutive columns of every row.  A per-wave LDS buffer [R][4][64] holds
+ * the group (slot t&3); one 16-byte access per row replaces four scattered
+ * 4-byte accesses, and only the lanes at a row's two ends fall back to
+ * single words. */
+typedef float f4u __attribute__((ext_vector_type(4), aligned(4)));
+
+/* row r = rbase + R*lane + k of a row-major [L1][L2] block starts at
+ * (rbase + k)*L2 (wave-uniform) + lane*R*L2 (lo, the lane's part), so the
+ * R row addresses share one per-lane register pair */
+template <int R>
+__device__ __forceinline__ void grp_load(const float* M, int L1, int L2, int rbase, int64_t lo,
+                                         int c0, float* buf, int lane)
+{
+    const bool full = c0 >= 0 && c0 + 3 < L2;
+#pragma unroll 1
+    for (int k = 0; k < R; k++) {
+        if (rbase + R * lane + k >= L1) continue;
+        const float* row = M + (int64_t)(rbase + k) * L2 + lo;
+        float v0 = 0.0f, v1 = 0.0f, v2 = 0.0f, v3 = 0.0f;
+        if (full) {
+            const f4u v = *(const f4u*)(row + c0);
+            v0 = v.x; v1 = v.y; v2 = v.z; v3 = v.w;
+        } else {
+            if (c0 >= 0 && c0 < L2) v0 = row[c0];
+            if (c0 + 1 >= 0 && c0 + 1 < L2) v1 = row[c0 + 1];
+            if (c0 + 2 >= 0 && c0 + 2 < L2) v2 = row[c0 + 2];
+            if (c0 + 3 >= 0 && c0 + 3 < L2) v3 = row[c0 + 3];
+        }
+        float* b = buf + k * 256 + lane;
+        b[0] = v0; b[64] = v1; b[128] = v2; b[192] = v3;
+    }
+}
+
+template <int R>
+__device__ __forceinline__ void grp_store(float* M, int L1, int L2, int rbase, int64_t lo,
+                                          int c0, const float* buf, int lane)
+{
+    const bool full = c0 >= 0 && c0 + 3 < L2;
+#pragma unroll 1
+    for (int k = 0; k < R; k++) {
+        if (rbase + R * lane + k >= L1) continue;
+        float* row = M + (int64_t)(rbase + k) * L2 + lo;
+        const float* b = buf + k * 256 + lane;
+        if (full) {
+            f4u v;
+            v.x = b[0]; v.y = b[64]; v.z = b[128]; v.w = b[192];
+            *(f4u*)(row + c0) = v;
+        } else {
+            if (c0 >= 0 && c0 < L2) row[c0] = b[0];
+            if (c0 + 1 >= 0 && c0 + 1 < L2) row[c0 + 1] = b[64];
+            if (c0 + 2 >= 0 && c0 + 2 < L2) row[c0 + 2] = b[128];
+            if (c0 + 3 >= 0 && c0 + 3 < L2) row[c0 + 3] = b[192];
+        }
+    }
+}
+
+/* LDS offset of the current step's word in a row's group (slot t & 3) */
+__device__ __forceinline__ int k_grp_slot(int t, int lane) { return (t & 3) * 64 + lane; }
+
+/* floats of one wave's group buffer */
+template <int R>
+__device__ __forceinline__ constexpr int grp_floats() { return R * 4 * 64; }
+
 __device__ __forceinline__ S5 fcell(const S5& dg, const S5& up, const S5& lf, const float* qt,
                                     const float* dt, float s, const Th& th)
 {
@@ -154,7 +220,7 @@ __device__ __forceinline__ S5 top_at(const SdpArgs& a, const SdpPair& P, const f
 template <int R>
 __device__ __forceinline__ void fwd_cells(const SdpArgs& a, const SdpPair& P, const Stripe<R>& sp,
                                           FwdState<R>& w, int s, int ns, int t, int lane, const Th& th,
-                                          const float* S, float* fhw, float* lds,
+                                          const float* sg, float* fhw, float* lds,
                                           float& zm, float& zs, bool accz)
 {
     const int c = t - lane;
@@ -174,7 +240,7 @@ __device__ __forceinline__ void fwd_cells(const SdpArgs& a, const SdpPair& P, co
             }
             const S5 up = (k == 0) ? w.U : w.H[k - 1];
             prevk = w.H[k];
-            const float sc = sp.valid[k] ? S[(size_t)sp.r[k] * P.L2 + c] * L2E : 0.0f;
+            const float sc = sp.valid[k] ? sg[k * 256 + (t & 3) * 64 + lane] * L2E : 0.0f;
             const S5 n = fcell(dg, up, lf, sp.qt[k], dt, sc, th);
             w.H[k] = n;
             if (accz && sp.valid[k]) {
@@ -213,10 +279,11 @@ __device__ __forceinline__ void fwd_shift(FwdState<R>& w, const S5& tp)
 template <int R>
 __device__ __forceinline__ void fwd_step(const SdpArgs& a, const SdpPair& P, const Stripe<R>& sp,
                                          FwdState<R>& w, int s, int ns, int t, int lane, const Th& th,
-                                         const float* S, const float* fh)
+                                         const float* S, const float* fh, float* sg)
 {
     float zm = 0.0f, zs = 0.0f;
-    fwd_cells<R>(a, P, sp, w, s, ns, t, lane, th, S, nullptr, nullptr, zm, zs, false);
+    if ((t & 3) == 0) grp_load<R>(S, P.L1, P.L2, sp.rbase, (int64_t)lane * R * P.L2, t - lane, sg, lane);
+    fwd_cells<R>(a, P, sp, w, s, ns, t, lane, th, sg, nullptr, nullptr, zm, zs, false);
     fwd_shift<R>(w, top_at(a, P, fh, s, t + 1));
 }
 
@@ -284,7 +351,9 @@ __global__ __launch_bounds__(512) void k_sdp_fwd(SdpArgs a)
 {
     __shared__ float hx[2][8][5];
     __shared__ float wz[8][2];
+    extern __shared__ float sdp_grp[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, NW = blockDim.x >> 6;
+    float* sg = sdp_grp + wv * grp_floats<R>();       /* this wave's S group */
     const Th th = load_theta(a.theta);
     const float lp0 = -99.0f * L2E;
     for (int slot = blockIdx.x; slot < a.npairs; slot += gridDim.x) {
@@ -312,7 +381,8 @@ __global__ __launch_bounds__(512) void k_sdp_fwd(SdpArgs a)
                 if (have && t >= 0 && t < nsteps) {
                     if (a.K > 0 && t > 0 && (t % a.K) == 0)
                         put_ckpt<R>(store + ((int64_t)s * P.ncp + t / a.K) * ckpt_floats<R>(), lane, w);
-                    fwd_cells<R>(a, P, sp, w, s, ns, t, lane, th, S, fh,
+                    if ((t & 3) == 0) grp_load<R>(S, P.L1, P.L2, sp.rbase, (int64_t)lane * R * P.L2, t - lane, sg, lane);
+                    fwd_cells<R>(a, P, sp, w, s, ns, t, lane, th, sg, fh,
                                  (wv + 1 < nwb) ? hx[T & 1][wv] : nullptr, zm, zs, true);
                     if (a.K == 0 && (t - lane) >= 0 && (t - lane) < P.L2)
                         put_cells<R>(store + ((int64_t)s * nsteps + t) * 5 * R * 64, lane, w.H);
@@ -373,7 +443,11 @@ __global__ __launch_bounds__(512) void k_sdp_adj(SdpArgs a)
 {
     __shared__ float hb[2][8][3];
     __shared__ double wacc[8][5];
+    extern __shared__ float sdp_grp[];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6, NW = blockDim.x >> 6;
+    float* sg = sdp_grp + (3 * wv + 0) * grp_floats<R>();   /* S group */
+    float* dg = sdp_grp + (3 * wv + 1) * grp_floats<R>();   /* dS group */
+    float* pg = sdp_grp + (3 * wv + 2) * grp_floats<R>();   /* posterior group */
     const int gw = blockIdx.x * NW + wv;
     const Th th = load_theta(a.theta);
     float* scr = a.ws + a.scr_off + (int64_t)gw * a.K * 5 * R * 64;
@@ -402,6 +476,7 @@ __global__ __launch_bounds__(512) void k_sdp_adj(SdpArgs a)
             const int nwb = min(NW, ns - b * NW);
             Stripe<R> sp;
             load_stripe<R>(a, P, s, lane, sp);
+            const int64_t lo = (int64_t)lane * R * L2;      /* the lane's part of its row offsets */
             float qn[7];                             /* transitions of the row below the lane's last row */
             const int rn = sp.rbase + R * (lane + 1);
             if (rn < P.L1) ld_trn(a.qtr, a.qntr, qb, rn, qn);
@@ -432,18 +507,25 @@ __global__ __launch_bounds__(512) void k_sdp_adj(SdpArgs a)
                         get_ckpt<R>(store + ((int64_t)s * P.ncp + gseg) * ckpt_floats<R>(), lane, w);
                     }
                     for (int t = t0; t < t1; t++) {
-                        fwd_step<R>(a, P, sp, w, s, ns, t, lane, th, S, fh);
+                        fwd_step<R>(a, P, sp, w, s, ns, t, lane, th, S, fh, sg);
                         if ((t - lane) >= 0 && (t - lane) < L2)
                             put_cells<R>(scr + (int64_t)(t - t0) * 5 * R * 64, lane, w.H);
                     }
                     __threadfence_block();
                 }
+                /* this phase's parameter-gradient sums in FP32 (at most KK*R terms per
+                 * lane), added to the pair's FP64 sums after the phase */
+                float f0 = 0.0f, f1 = 0.0f, f2 = 0.0f, f3 = 0.0f, f4 = 0.0f;
                 for (int j = 0; j < KK; j++) {
                     const int T = ph * KK + j;
                     const int t = t0 + KK - 1 - j;
                     if (segok && t < nsteps) {
                         const int c = t - lane;
                         const bool act = (c >= 0) && (c < L2);
+                        /* the reverse sweep enters a group of 4 steps at its top step */
+                        if ((t & 3) == 3 || t == nsteps - 1)
+                            grp_load<R>(S, P.L1, L2, sp.rbase, lo, (t & ~3) - lane, sg, lane);
+                        const int gs = k_grp_slot(t, lane);
                         float dc[7];
                         if (act) ld_trn(a.dtr, a.dntr, db, c, dc);
                         else for (int jj = 0; jj < 7; jj++) dc[jj] = 0.0f;
@@ -493,20 +575,19 @@ __global__ __launch_bounds__(512) void k_sdp_adj(SdpArgs a)
                                     const float bnd = mn + lg2(fn1 + fn2);
                                     S5 f;
                                     get_cell<R>(fb, lane, k, f);
-                                    const size_t ci = (size_t)sp.r[k] * L2 + c;
                                     const float wz = ex2(bmm - Z);
                                     /* e^{F_MM + B_MM - Z} = e^{F_MM + mm - Z} * sum6 */
                                     const float wm = ex2(f.mm + mm - Z), wi = ex2(f.mi + mi - Z), wj = ex2(f.im + mj - Z);
                                     const float pmm = wm * sum6;
-                                    dS[ci] = (pmm - wz) * g;
-                                    if (post) post[ci] = pmm;
+                                    dg[k * 256 + gs] = (pmm - wz) * g;
+                                    if (post) pg[k * 256 + gs] = pmm;
                                     const float wd = ex2(f.dn + md - Z), wn = ex2(f.nd + mn - Z);
-                                    acc0 += (double)(wm * (e2 + e3));
-                                    acc2 += (double)(wm * (e4 + e5));
-                                    acc1 += (double)(wi * fi2 + wj * fj2);
-                                    acc3 += (double)(wd * fd2 + wn * fn2);
-                                    acc4 += (double)(wi * fi1 + wj * fj1 + wd * fd1 + wn * fn1);
-                                    const float sc = S[ci] * L2E;
+                                    f0 = fmaf(wm, e2 + e3, f0);
+                                    f2 = fmaf(wm, e4 + e5, f2);
+                                    f1 = fmaf(wi, fi2, fmaf(wj, fj2, f1));
+                                    f3 = fmaf(wd, fd2, fmaf(wn, fn2, f3));
+                                    f4 = fmaf(wi, fi1, fmaf(wj, fj1, fmaf(wd, fd1, fmaf(wn, fn1, f4))));
+                                    const float sc = sg[k * 256 + gs] * L2E;
                                     if (c == 0 || sp.r[k] == 0) {
                                         /* transitions whose source is the row -1 / column -1
                                          * boundary: counted here at their destination */
@@ -519,18 +600,18 @@ __global__ __launch_bounds__(512) void k_sdp_adj(SdpArgs a)
                                             lf0.dn = a.qll[bi + 3] * L2E; lf0.nd = a.qll[bi + 4] * L2E;
                                             dg0.mm = a.qld[bi] * L2E; dg0.mi = a.qld[bi + 1] * L2E; dg0.im = a.qld[bi + 2] * L2E;
                                             dg0.dn = a.qld[bi + 3] * L2E; dg0.nd = a.qld[bi + 4] * L2E;
-                                            acc0 += (double)ex2(lf0.mm + qs[1] + dc[0] + th.oi + bim - Z);
-                                            acc1 += (double)ex2(lf0.im + qs[4] + dc[0] + th.ei + bim - Z);
-                                            acc2 += (double)ex2(lf0.mm + dc[2] + th.od + bnd - Z);
-                                            acc3 += (double)ex2(lf0.nd + dc[6] + th.ed + bnd - Z);
+                                            f0 += ex2(lf0.mm + qs[1] + dc[0] + th.oi + bim - Z);
+                                            f1 += ex2(lf0.im + qs[4] + dc[0] + th.ei + bim - Z);
+                                            f2 += ex2(lf0.mm + dc[2] + th.od + bnd - Z);
+                                            f3 += ex2(lf0.nd + dc[6] + th.ed + bnd - Z);
                                         }
                                         if (sp.r[k] == 0) {
-                                            acc0 += (double)ex2(up0.mm + qs[0] + dc[1] + th.oi + bmi - Z);
-                                            acc1 += (double)ex2(up0.mi + qs[0] + dc[4] + th.ei + bmi - Z);
-                                            acc2 += (double)ex2(up0.mm + qs[2] + th.od + bdn - Z);
-                                            acc3 += (double)ex2(up0.dn + qs[6] + th.ed + bdn - Z);
+                                            f0 += ex2(up0.mm + qs[0] + dc[1] + th.oi + bmi - Z);
+                                            f1 += ex2(up0.mi + qs[0] + dc[4] + th.ei + bmi - Z);
+                                            f2 += ex2(up0.mm + qs[2] + th.od + bdn - Z);
+                                            f3 += ex2(up0.dn + qs[6] + th.ed + bdn - Z);
                                         }
-                                        acc4 += (double)(ex2(dg0.mi + qs[0] + dc[3] + th.cl + sc + bmm - Z) +
+                                        f4 += (ex2(dg0.mi + qs[0] + dc[3] + th.cl + sc + bmm - Z) +
                                                          ex2(dg0.im + qs[3] + dc[0] + th.cl + sc + bmm - Z) +
                                                          ex2(dg0.dn + qs[5] + dc[0] + th.cl + sc + bmm - Z) +
                                                          ex2(dg0.nd + qs[0] + dc[5] + th.cl + sc + bmm - Z));
@@ -554,6 +635,11 @@ __global__ __launch_bounds__(512) void k_sdp_adj(SdpArgs a)
                             for (int k = 0; k < R; k++) cur[k] = s5_fill(NEG, NEG);
                         }
                         for (int jj = 0; jj < 7; jj++) dx[jj] = dc[jj];
+                        /* the group's last step: its four columns go out */
+                        if ((t & 3) == 0) {
+                            grp_store<R>(dS, P.L1, L2, sp.rbase, lo, t - lane, dg, lane);
+                            if (post) grp_store<R>(post, P.L1, L2, sp.rbase, lo, t - lane, pg, lane);
+                        }
                     }
                     __syncthreads();
                     if (segok && t >= 1 && t <= nsteps) {
@@ -575,6 +661,8 @@ __global__ __launch_bounds__(512) void k_sdp_adj(SdpArgs a)
                         sh.dn = wave_shl1(cur[0].dn, ob.dn);
                     }
                 }
+                acc0 += (double)f0; acc1 += (double)f1; acc2 += (double)f2;
+                acc3 += (double)f3; acc4 += (double)f4;
             }
             __syncthreads();
         }
@@ -650,8 +738,18 @@ hipError_t c2_launch_sdp(const SdpArgs& a, bool adjoint, int groups, int nw, hip
     if (a.npairs <= 0) return hipSuccess;
     if (groups < 1 || nw < 1 || nw > 8) return hipErrorInvalidValue;
     if (a.K < 0 || (a.K > 0 && 64 % a.K != 0)) return hipErrorInvalidValue;
-    if (adjoint) hipLaunchKernelGGL((k_sdp_adj<5>), dim3(groups), dim3(64 * nw), 0, stream, a);
-    else hipLaunchKernelGGL((k_sdp_fwd<5>), dim3(groups), dim3(64 * nw), 0, stream, a);
+    const size_t grp = sizeof(float) * 4 * 64 * 5;      /* one wave's [R=5][4][64] group buffer */
+    static bool attr = false;
+    if (!attr) {
+        /* up to 8 waves x 3 group buffers (120 KiB) of dynamic LDS */
+        hipError_t e = hipFuncSetAttribute((const void*)k_sdp_adj<5>, hipFuncAttributeMaxDynamicSharedMemorySize, 24 * (int)grp);
+        if (e == hipSuccess)
+            e = hipFuncSetAttribute((const void*)k_sdp_fwd<5>, hipFuncAttributeMaxDynamicSharedMemorySize, 8 * (int)grp);
+        if (e != hipSuccess) return e;
+        attr = true;
+    }
+    if (adjoint) hipLaunchKernelGGL((k_sdp_adj<5>), dim3(groups), dim3(64 * nw), 3 * nw * grp, stream, a);
+    else hipLaunchKernelGGL((k_sdp_fwd<5>), dim3(groups), dim3(64 * nw), nw * grp, stream, a);
     return hipGetLastError();
 }
 
