@@ -37,8 +37,8 @@ namespace {
 constexpr int RING = 96;      /* columns in the per-wave LDS ring */
 constexpr int BLK = 32;       /* columns loaded per refill */
 /* ring field offsets (each field is RING words) */
-constexpr int F_DW = 0, F_DT = 20, F_PJ = 27, F_X = 30, F_LEN = 31, F_PID = 32, F_TOP = 33;
-constexpr int NF_BASE = 33, NF_TOP = 38;
+constexpr int F_DW = 0, F_DT = 20, F_PJ = 27, F_X = 30, F_LEN = 31, F_PID = 32, F_BIN = 33, F_TOP = 34;
+constexpr int NF_BASE = 34, NF_TOP = 39;
 
 /* tops: the ring carries the stripe's top boundary; rtop: read it from the
  * hand-off buffer (stripes after the first) instead of the zero row -1 */
@@ -95,6 +95,10 @@ __device__ __forceinline__ void load_block(PA& a, float* ring, int lane, int t0,
         ring[F_X * RING + slot] = __int_as_float(x);
         ring[F_LEN * RING + slot] = __int_as_float(a.len[p]);
         ring[F_PID * RING + slot] = __int_as_float(p);
+        /* the column's SS-table bins int(9.999 p'_j), 4 bits each */
+        const uint32_t bins = (uint32_t)(int)(sp[0] * 9.999f) | ((uint32_t)(int)(sp[(size_t)npos] * 9.999f) << 4) |
+                              ((uint32_t)(int)(sp[2 * (size_t)npos] * 9.999f) << 8);
+        ring[F_BIN * RING + slot] = __uint_as_float(bins);
         if (tops) {
             const float* bt = bot + (c - bc0);
             for (int s = 0; s < 5; s++)
@@ -107,6 +111,7 @@ __device__ __forceinline__ void load_block(PA& a, float* ring, int lane, int t0,
         ring[F_X * RING + slot] = __int_as_float(-1);
         ring[F_LEN * RING + slot] = __int_as_float(1);
         ring[F_PID * RING + slot] = __int_as_float(-1);
+        ring[F_BIN * RING + slot] = 0.0f;
         if (tops)
             for (int s = 0; s < 5; s++) ring[(F_TOP + s) * RING + slot] = 0.0f;
     }
@@ -183,6 +188,7 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
             ring[F_X * RING + i] = __int_as_float(-1);
             ring[F_LEN * RING + i] = __int_as_float(1);
             ring[F_PID * RING + i] = __int_as_float(-1);
+            ring[F_BIN * RING + i] = 0.0f;
             if (tops) for (int j = 0; j < 5; j++) ring[(F_TOP + j) * RING + i] = 0.0f;
         }
         wave_lds_sync();
@@ -258,8 +264,11 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
 
             /* per-column SS table offsets: QT[10j + int(9.999 p'_j)] */
             int co[3];
+            {
+                const uint32_t bb = __float_as_uint(ring[F_BIN * RING + slot]);
 #pragma unroll
-            for (int j = 0; j < 3; j++) co[j] = (10 * j + (int)(pj[j] * 9.999f)) * SL + R * lane;
+                for (int j = 0; j < 3; j++) co[j] = (10 * j + (int)((bb >> (4 * j)) & 15u)) * SL + R * lane;
+            }
             const float dh = (pj[2] + 0.1f) * 0.1111111f;
             const int u0 = (xloc - y0) - cA - 1;
 
@@ -289,8 +298,21 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
                 saved = H[k];
                 const S5 n = dp_cell(dg, up, H[k], qt[k], dt, sc);
                 H[k] = n;
-                if (n.mm > best) { best = n.mm; bpos = (t << 3) | k; }
                 if (BTCK) bw |= dp_btck(n) << (2 * k);
+            }
+            /* running maximum: the step's largest cell, the first row on ties
+             * (the order of a row-by-row strict-greater update) */
+            {
+                float m = H[0].mm;
+#pragma unroll
+                for (int k = 1; k < R; k++) m = fmaxf(m, H[k].mm);
+                if (m > best) {
+                    int kb = R - 1;
+#pragma unroll
+                    for (int k = R - 2; k >= 0; k--) if (H[k].mm == m) kb = k;
+                    best = m;
+                    bpos = (t << 3) | kb;
+                }
             }
             if (BTCK) btw[(size_t)t * 64 + lane] = bw;
             if (wbot && lane == 63) {
