@@ -165,6 +165,18 @@ __global__ __launch_bounds__(256) void k_map_score(ScoreCtx c, MapArgs m, int ni
     }
     const bool dynmod = (mp.dmul != 0.0f && mp.modinuse);
     float* Sd = m.Sd + mp.foff;
+    /* the stripe's rows of the 30-entry SS tables in LDS (one slice per
+     * wave) for R <= 2, so the column's table lookups are not a second,
+     * dependent global load; wider stripes read them from HBM */
+    constexpr bool QT_LDS = (R <= 2);
+    constexpr int SLR = 64 * R;
+    __shared__ float sqt[QT_LDS ? 4 * 30 * SLR : 1];
+    float* qs = sqt + (QT_LDS ? (threadIdx.x >> 6) * 30 * SLR : 0);
+    if constexpr (QT_LDS) {
+        for (int i = lane; i < 30 * SLR; i += 64)
+            qs[i] = c.QT[(size_t)(i / SLR) * c.Lpad + s * SLR + (i % SLR)];
+        wave_lds_sync();
+    }
     /* column data of the next step (and the rows' SS table entries it
      * selects) are loaded one step ahead */
     float dn[20], pn[3], qtn[R][3];
@@ -178,7 +190,10 @@ __global__ __launch_bounds__(256) void k_map_score(ScoreCtx c, MapArgs m, int ni
             pn[j] = ok ? c.ssprb[(size_t)j * c.npos + col] : 0.0f;
             const int cj = 10 * j + (int)(pn[j] * 9.999f);
 #pragma unroll
-            for (int k = 0; k < R; k++) qtn[k][j] = (ok && valid[k]) ? c.QT[(size_t)cj * c.Lpad + y[k]] : 0.0f;
+            for (int k = 0; k < R; k++) {
+                if constexpr (QT_LDS) qtn[k][j] = (ok && valid[k]) ? qs[cj * SLR + R * lane + k] : 0.0f;
+                else qtn[k][j] = (ok && valid[k]) ? c.QT[(size_t)cj * c.Lpad + y[k]] : 0.0f;
+            }
         }
     };
     fetch(-lane);
@@ -386,14 +401,44 @@ __global__ __launch_bounds__(256) void k_map_total(ScoreCtx c, MapArgs m)
             live = !(y < 0 || Lq <= y + 1 - 32 || Ld <= x0 || ilim < 1);
         }
         if (live) {
-            for (int i = 0; i < 32; i++) {
-                const int yy = y - i, xx = x0 + t + i;
-                float nv = (0 <= yy && yy < Lq && 0 <= xx && xx < Ld) ? mexp(F[cell_lsi<R>(yy, xx, nsteps)] - maxF) : 0.0f;
-                for (int off = 16; off >= 1; off >>= 1) {
-                    const float o = __shfl_down(nv, off, 32);
-                    if (t + off < 32) nv += o;
+            if constexpr (R == 1) {
+                /* one row per lane: the block's rows y-31..y are lanes ltop-31..ltop
+                 * of one stripe, and lane t's cells (y-i, x0+t+i) all sit at step
+                 * x0+t+ltop, 32 consecutive words: one 128-byte read per lane
+                 * instead of 32 scattered ones (same values, same sums) */
+                const int sI = y >> 6, ltop = y & 63, step = x0 + t + ltop;
+                float v[32];
+                if (step >= 0 && step < nsteps) {
+                    const float4* src = (const float4*)(F + ((int64_t)sI * nsteps + step) * 64 + (ltop - 31));
+#pragma unroll
+                    for (int q = 0; q < 8; q++) {
+                        const float4 w4 = src[q];
+                        v[4 * q] = w4.x; v[4 * q + 1] = w4.y; v[4 * q + 2] = w4.z; v[4 * q + 3] = w4.w;
+                    }
+                } else {
+#pragma unroll
+                    for (int q = 0; q < 32; q++) v[q] = 0.0f;
                 }
-                sum += nv;
+#pragma unroll
+                for (int i = 0; i < 32; i++) {
+                    const int yy = y - i, xx = x0 + t + i;
+                    float nv = (0 <= yy && yy < Lq && 0 <= xx && xx < Ld) ? mexp(v[31 - i] - maxF) : 0.0f;
+                    for (int off = 16; off >= 1; off >>= 1) {
+                        const float o = __shfl_down(nv, off, 32);
+                        if (t + off < 32) nv += o;
+                    }
+                    sum += nv;
+                }
+            } else {
+                for (int i = 0; i < 32; i++) {
+                    const int yy = y - i, xx = x0 + t + i;
+                    float nv = (0 <= yy && yy < Lq && 0 <= xx && xx < Ld) ? mexp(F[cell_lsi<R>(yy, xx, nsteps)] - maxF) : 0.0f;
+                    for (int off = 16; off >= 1; off >>= 1) {
+                        const float o = __shfl_down(nv, off, 32);
+                        if (t + off < 32) nv += o;
+                    }
+                    sum += nv;
+                }
             }
         }
         if (t == 0) bsum[g] = sum;
