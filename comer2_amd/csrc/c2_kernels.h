@@ -94,7 +94,14 @@ struct MapArgs {
     int32_t R;                       /* query rows per lane of the MAP passes */
     int32_t ns;                      /* stripes of 64*R rows */
     int32_t nw;                      /* waves per workgroup (stripes in flight per pair) */
+    /* work items of the dynamic-score pass: (pair, stripe, first column, end
+     * column), longest pairs first; columns are independent there, so long
+     * pairs are cut into column chunks */
+    const int4* sitems;
+    int32_t nsitems;
 };
+/* columns per item of the dynamic-score pass */
+constexpr int C2_MAP_SCORE_COLS = 256;
 /* MAP geometry for a query: rows per lane and waves per pair */
 void c2_map_geometry(int Lq, int* R, int* ns, int* nw);
 hipError_t c2_launch_map(const ScoreCtx& c, const MapArgs& m, int npairs, hipStream_t stream);

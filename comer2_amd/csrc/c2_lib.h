@@ -69,8 +69,9 @@ struct QWork {
     std::vector<int> sel;
     std::vector<MapPair> mp;
     std::vector<int32_t> morder;
+    std::vector<int4> msitems;          /* dynamic-score work items */
     int64_t alnstride = 0;
-    DBuf mpairs, dmorder, mF, mSd, mhand, mbt, mstat, malns, malntmp, mres;
+    DBuf mpairs, dmorder, dmsitems, mF, mSd, mhand, mbt, mstat, malns, malntmp, mres;
     std::vector<int32_t> hmres;
     int64_t nposits = 0;
     int32_t maxlen = 0;

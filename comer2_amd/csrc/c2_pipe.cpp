@@ -487,10 +487,20 @@ int stage_map(cbp_handle* h, QWork& w)
     std::stable_sort(w.morder.begin(), w.morder.end(), [&](int a, int b) {
         return ch.len[w.mp[a].pro] > ch.len[w.mp[b].pro];
     });
+    /* dynamic-score items: (pair, stripe, column chunk), longest pairs first */
+    w.msitems.clear();
+    for (int o = 0; o < nmap; o++) {
+        const int k = w.morder[o];
+        const int ld = ch.len[w.mp[k].pro];
+        for (int st = 0; st < mns; st++)
+            for (int c0 = 0; c0 < ld; c0 += C2_MAP_SCORE_COLS)
+                w.msitems.push_back(int4{k, st, c0, std::min(ld, c0 + C2_MAP_SCORE_COLS)});
+    }
     w.alnstride = (int64_t)dq->hp.Lq + maxld + 1;
     int rc;
     if ((rc = c2_dupload(h, w.mpairs, w.mp.data(), nmap, w.st))) return rc;
     if ((rc = c2_dupload(h, w.dmorder, w.morder.data(), nmap, w.st))) return rc;
+    if ((rc = c2_dupload(h, w.dmsitems, w.msitems.data(), w.msitems.size(), w.st))) return rc;
     if ((rc = c2_dalloc(h, w.mF, sizeof(float) * (size_t)foff))) return rc;
     if ((rc = c2_dalloc(h, w.mSd, sizeof(float) * (size_t)foff))) return rc;
     if ((rc = c2_dalloc(h, w.mhand, sizeof(float) * (size_t)hoff))) return rc;
@@ -512,6 +522,7 @@ int stage_map(cbp_handle* h, QWork& w)
     ma.alns = w.malns.as<char>(); ma.alntmp = w.malntmp.as<char>(); ma.alnstride = w.alnstride;
     ma.res = w.mres.as<int32_t>(); ma.minpp = h->opt.minpp;
     ma.order = w.dmorder.as<int32_t>(); ma.R = mR; ma.ns = mns; ma.nw = mnw;
+    ma.sitems = w.dmsitems.as<int4>(); ma.nsitems = (int32_t)w.msitems.size();
     /* strings are NUL-padded to the stride: the whole buffer is output */
     PCHK(c2_launch_zero_u32(w.malns.as<uint32_t>(), (int64_t)alnwords, w.st));
     PCHK(c2_launch_map(sc, ma, nmap, w.st));
