@@ -101,3 +101,36 @@ def test_map_edge_lengths():
             assert res["alns"][k] == _strings(aln), ctx
             n += 1
     assert n > 0
+
+
+def test_map_score_chunk_edges():
+    """The dynamic-score pass cuts a profile into 256-column items: profiles
+    at and around one and two chunk widths give the oracle's alignments."""
+    EQ = [40, 100]
+    ED = [255, 256, 257, 511, 512, 513, 700]
+    q = SynthDB.from_lengths(131, EQ)
+    d = SynthDB.from_lengths(132, ED)
+    orc = oracle()
+    cbp = processing(dict(max_map_pairs=0))
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    n = 0
+    seen = set()
+    for qp in range(q.nprof):
+        print(f"map chunk edges: Lq={EQ[qp]}", flush=True)
+        res = cbp.ProcessScoreMatrix(qp, scorethld=-1.0, logevthld=50.0)
+        sel = [int(i) for i in res["order"] if res["logE"][i] <= 50.0]
+        assert list(res["map_rows"]) == sel
+        for k, row in enumerate(sel):
+            dp = int(res["pronr"][row])
+            S, M = orc.score_f32(q.view, qp, d.view, dp)
+            r, _, _, _, aln = orc.map(q.view, qp, d.view, dp, S, M, np.float32(res["epa"][row]))
+            ctx = (EQ[qp], ED[dp])
+            assert res["alnlen"][row] == r.alnlen, ctx
+            assert res["beg"][row] == ((r.begx << 16) | r.begy), ctx
+            assert res["end"][row] == ((r.x << 16) | r.y), ctx
+            assert res["alns"][k] == _strings(aln), ctx
+            n += 1
+            seen.add(ED[dp])
+    assert n > 0 and seen == set(ED)
