@@ -158,9 +158,19 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
 
         const int pb = a.seg_pb[seg], pe = a.seg_pe[seg];
         const int64_t c_begin = a.dist[pb];
-        const int64_t c_end = (int64_t)a.dist[pe - 1] + a.len[pe - 1];
+        const int64_t c_full = (int64_t)a.dist[pe - 1] + a.len[pe - 1];
+        int64_t c_end = c_full;
+        if (BTCK) {
+            /* one candidate profile: the backtrace starts at the maximum's
+             * column x (the key phase 1 left), so columns past x are not swept;
+             * the codes keep the whole profile's stride */
+            const uint32_t xy = 0xFFFFFFFFu - (uint32_t)a.keys[pb];
+            const int64_t cut = c_begin + (int64_t)(xy >> 16) + 1;
+            if (cut < c_end) c_end = cut;
+        }
         const int ncols = (int)(c_end - c_begin);
         const int nsteps = ncols + 63;
+        const int bstride = (int)(c_full - c_begin) + 63;   /* backtrack-code steps per stripe */
         const int y0 = rbase + R * lane;
         float* bot = a.bot;
         int64_t bld = a.npos, bc0 = 0;
@@ -209,7 +219,7 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
         int bpos = 0, curpid = -1, pstart = 0, cA = 0, cW = 0;
         const bool wbot = tops && (s + 1 < a.nstripes);
         uint32_t* btw = nullptr;
-        if (BTCK) btw = a.btck + a.btck_off[seg] + (size_t)s * nsteps * 64;
+        if (BTCK) btw = a.btck + a.btck_off[seg] + (size_t)s * bstride * 64;
 
         for (int t = 0; t < nsteps; t++) {
             if (((t + 1) % BLK) == 0) {
