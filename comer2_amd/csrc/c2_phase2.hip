@@ -98,25 +98,32 @@ constexpr int NSC = 641;
 
 /* query rows staged per tile: 20 target frequencies + p_H + 0.1 as 21
  * words per row (broadcast reads), and the 30-entry SS table per row */
-constexpr int HTY = 256;
+constexpr int HTY = 128;
 constexpr int HQW = 21;
 
 __global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, int xuninf, int32_t* counts)
 {
-    __shared__ int h[NSC];
+    constexpr int NWH = 4;                  /* waves of the 256-thread workgroup, one histogram each */
+    __shared__ int h[NWH][NSC];
     __shared__ float sq[HTY * HQW];
     __shared__ float sqt[30 * HTY];
     __shared__ unsigned char sx[HTY];
-    for (int i = threadIdx.x; i < NSC; i += blockDim.x) h[i] = 0;
+    for (int i = threadIdx.x; i < NWH * NSC; i += blockDim.x) (&h[0][0])[i] = 0;
+    int* hw = h[threadIdx.x >> 6];
     const int p = pro[blockIdx.x];
     const int Ldb = c.len[p];
     const int64_t col0 = c.dist[p];
     float w[20];
     c2_pair_weights(c, p, w);
-    for (int x0 = 0; x0 < Ldb; x0 += blockDim.x) {
-        const int x = x0 + threadIdx.x;
+    /* a profile shorter than the workgroup is covered several times over:
+     * thread (phase ph, column xc) takes the rows r = ph (mod nph) */
+    const int Lw = Ldb < (int)blockDim.x ? Ldb : (int)blockDim.x;
+    const int nph = (int)blockDim.x / Lw;
+    const int ph = (int)threadIdx.x / Lw, xc = (int)threadIdx.x - ph * Lw;
+    for (int x0 = 0; x0 < Ldb; x0 += Lw) {
+        const int x = x0 + xc;
         const int64_t col = col0 + (x < Ldb ? x : 0);
-        const bool xok = x < Ldb && !(xuninf && c.aa[col] == 'X');
+        const bool xok = ph < nph && x < Ldb && !(xuninf && c.aa[col] == 'X');
         float dw[20], pj[3];
         int co[3];
 #pragma unroll
@@ -142,7 +149,7 @@ __global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, in
                 sx[i] = (xuninf && c.qaa[y0 + i] == 'X') ? 1 : 0;
             __syncthreads();
             if (!xok) continue;
-            for (int r = 0; r < ny; r++) {
+            for (int r = ph; r < ny; r += nph) {
                 if (sx[r]) continue;
                 const float* qr = sq + r * HQW;
                 float acc = qr[0] * dw[0];
@@ -159,13 +166,13 @@ __global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, in
                 }
                 if (sc < -10.0f) sc = -10.0f; else if (sc > 10.0f) sc = 10.0f;
                 const int bin = (int)rintf(sc * 32.0f) + 320;
-                atomicAdd(&h[bin], 1);
+                atomicAdd(&hw[bin], 1);
             }
         }
     }
     __syncthreads();
     for (int i = threadIdx.x; i < NSC; i += blockDim.x)
-        counts[(size_t)blockIdx.x * NSC + i] = h[i];
+        counts[(size_t)blockIdx.x * NSC + i] = h[0][i] + h[1][i] + h[2][i] + h[3][i];
 }
 
 /* ---------------- FP64 statistics (one wave per pair) ---------------- */
