@@ -80,12 +80,14 @@ struct QWork {
     /* results */
     std::vector<float> aln, epa;
     std::vector<int32_t> order;
-    std::vector<char> strs;
+    char* strs = nullptr;               /* pinned host copy of the alignment strings */
+    size_t strs_cap = 0;
     std::vector<int> map;
     ~QWork() {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (hkeys) (void)hipHostFree(hkeys);
+        if (strs) (void)hipHostFree(strs);
     }
 };
 

@@ -527,9 +527,16 @@ int stage_map(cbp_handle* h, QWork& w)
     PCHK(c2_launch_zero_u32(w.malns.as<uint32_t>(), (int64_t)alnwords, w.st));
     PCHK(c2_launch_map(sc, ma, nmap, w.st));
     w.hmres.resize((size_t)8 * nmap);
-    w.strs.resize((size_t)nmap * 3 * w.alnstride);
+    /* the strings (NUL-padded to the stride, tens of MB for thousands of
+     * pairs) come back through pinned memory in one transfer */
+    const size_t nstr = (size_t)nmap * 3 * w.alnstride;
+    if (w.strs_cap < nstr) {
+        if (w.strs) { (void)hipHostFree(w.strs); w.strs = nullptr; w.strs_cap = 0; }
+        PCHK(hipHostMalloc((void**)&w.strs, nstr, hipHostMallocDefault));
+        w.strs_cap = nstr;
+    }
     PCHK(hipMemcpyAsync(w.hmres.data(), w.mres.p, sizeof(int32_t) * w.hmres.size(), hipMemcpyDeviceToHost, w.st));
-    PCHK(hipMemcpyAsync(w.strs.data(), w.malns.p, w.strs.size(), hipMemcpyDeviceToHost, w.st));
+    PCHK(hipMemcpyAsync(w.strs, w.malns.p, nstr, hipMemcpyDeviceToHost, w.st));
     return CBP_OK;
 }
 
@@ -556,7 +563,7 @@ void stage_results(cbp_handle* h, QWork& w, cbp_results* out)
     out->nmap = nmap;
     out->alndata = npass ? w.aln.data() : nullptr;
     out->order = npass ? w.order.data() : nullptr;
-    out->alns = nmap ? w.strs.data() : nullptr;
+    out->alns = nmap ? w.strs : nullptr;
     out->alnstride = w.alnstride;
     out->map_rows = nmap ? w.map.data() : nullptr;
     out->epa = npass ? w.epa.data() : nullptr;
