@@ -386,10 +386,11 @@ template <int R>
 __global__ __launch_bounds__(256) void k_map_total(ScoreCtx c, MapArgs m)
 {
     __shared__ float bsum[8];
-    const MapPair mp = m.pairs[blockIdx.x];
+    const int pi = m.order[blockIdx.x];
+    const MapPair mp = m.pairs[pi];
     const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63;
     const float* F = m.F + mp.foff;
-    const float maxF = m.stat[blockIdx.x * 4 + 0];
+    const float maxF = m.stat[pi * 4 + 0];
     const int nblkdiags = ((Ld + Lq) + 31) / 32 + (Lq - 1) / 32;
     const int lastdiag = 2 * ((Lq - 1) / 32);
     int maxblk = (Lq + 31) / 32;
@@ -457,7 +458,7 @@ __global__ __launch_bounds__(256) void k_map_total(ScoreCtx c, MapArgs m)
         }
         __syncthreads();
     }
-    if (threadIdx.x == 0) m.stat[blockIdx.x * 4 + 1] = mlog(total) + maxF;   /* logZ */
+    if (threadIdx.x == 0) m.stat[pi * 4 + 1] = mlog(total) + maxF;   /* logZ */
 }
 
 /* ---- backward: F := F + B_MM (in place) ----
@@ -708,8 +709,9 @@ __global__ __launch_bounds__(512) void k_map_dp(ScoreCtx c, MapArgs m)
 template <int R>
 __global__ __launch_bounds__(64) void k_map_final(ScoreCtx c, MapArgs m, int npairs)
 {
-    const int pi = blockIdx.x * blockDim.x + threadIdx.x;
-    if (pi >= npairs) return;
+    const int oi = blockIdx.x * blockDim.x + threadIdx.x;
+    if (oi >= npairs) return;
+    const int pi = m.order[oi];
     const MapPair mp = m.pairs[pi];
     const int p = mp.pro, Ld = c.len[p], nsteps = Ld + 63;
     const char* bt = m.btck + mp.boff;

@@ -423,7 +423,7 @@ int stage_stats(cbp_handle* h, QWork& w)
 }
 
 /* 4. statistics into the records, order, MAP selection and launch */
-int stage_map(cbp_handle* h, QWork& w)
+int stage_map(cbp_handle* h, QWork& w, bool split)
 {
     DevQuery* dq = w.dq;
     const cbp_profiles& ch = h->chunk.v;
@@ -525,7 +525,35 @@ int stage_map(cbp_handle* h, QWork& w)
     ma.sitems = w.dmsitems.as<int4>(); ma.nsitems = (int32_t)w.msitems.size();
     /* strings are NUL-padded to the stride: the whole buffer is output */
     PCHK(c2_launch_zero_u32(w.malns.as<uint32_t>(), (int64_t)alnwords, w.st));
-    PCHK(c2_launch_map(sc, ma, nmap, w.st));
+    /* every MAP pass is as long as its longest pair's column sweep; for a
+     * query on its own, the pairs longer than half the longest run their
+     * passes on a second stream, so the other pairs' passes fill the device
+     * meanwhile (per-pair outputs go to the same slots either way) */
+    int nA = 0;
+    if (split && nmap >= 2) {
+        const int lmax = ch.len[w.mp[w.morder[0]].pro];
+        while (nA < nmap - 1 && 2 * ch.len[w.mp[w.morder[nA]].pro] > lmax) nA++;
+    }
+    if (nA > 0) {
+        int64_t itA = 0;
+        for (int o = 0; o < nA; o++)
+            itA += (int64_t)mns * ((ch.len[w.mp[w.morder[o]].pro] + C2_MAP_SCORE_COLS - 1) / C2_MAP_SCORE_COLS);
+        if ((rc = ensure_fork_events(h))) return rc;
+        hipStream_t sA = h->xstream[0];
+        MapArgs mA = ma, mB = ma;
+        mA.nsitems = (int32_t)itA;
+        mB.order = ma.order + nA;
+        mB.sitems = ma.sitems + itA;
+        mB.nsitems = ma.nsitems - (int32_t)itA;
+        PCHK(hipEventRecord(h->gfork, w.st));
+        PCHK(hipStreamWaitEvent(sA, h->gfork, 0));
+        PCHK(c2_launch_map(sc, mA, nA, sA));
+        PCHK(c2_launch_map(sc, mB, nmap - nA, w.st));
+        PCHK(hipEventRecord(h->gjoin[0], sA));
+        PCHK(hipStreamWaitEvent(w.st, h->gjoin[0], 0));
+    } else {
+        PCHK(c2_launch_map(sc, ma, nmap, w.st));
+    }
     w.hmres.resize((size_t)8 * nmap);
     /* the strings (NUL-padded to the stride, tens of MB for thousands of
      * pairs) come back through pinned memory in one transfer */
@@ -700,7 +728,7 @@ int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* 
     if ((rc = sync_slots(h, nslots))) return rc;
     for (int i = 0; i < nq; i++) if ((rc = stage_stats(h, *h->work[i]))) return rc;
     if ((rc = sync_slots(h, nslots))) return rc;
-    for (int i = 0; i < nq; i++) if ((rc = stage_map(h, *h->work[i]))) return rc;
+    for (int i = 0; i < nq; i++) if ((rc = stage_map(h, *h->work[i], nq == 1))) return rc;
     if ((rc = sync_slots(h, nslots))) return rc;
     for (int i = 0; i < nq; i++) stage_results(h, *h->work[i], &outs[i]);
     /* device time of the phase-1 launches (the whole graph when replayed) */
