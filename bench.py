@@ -121,6 +121,13 @@ def batch_bench(d, args):
     qs = list(range(nq))
     lens = q.lengths()
     cells = int(lens.sum()) * d.npos
+    # launch geometry from the autotuner, timed on the median-length query
+    mid = int(np.argsort(lens, kind="stable")[nq // 2])
+    t = time.perf_counter()
+    seg_cols, nw, rmax = cbp.autotune(mid)
+    ttune = time.perf_counter() - t
+    log("bench: autotuned on query %d (L=%d): %d columns per segment, %d waves per workgroup, "
+        "at most %d rows per lane (%.1f s)" % (mid, int(lens[mid]), seg_cols, nw, rmax, ttune))
     cbp.ProcessBatch(qs[:4])                          # warm-up: buffers, query tables
     t = time.perf_counter()
     plain = [cbp.ProcessScoreMatrix(i) for i in qs]
@@ -160,7 +167,9 @@ def batch_bench(d, args):
            "queue_phase1_ms": round(qms, 3), "queue_phase1_gcups": round(cells / (qms * 1e-3) / 1e9, 3),
            "ratio": round(tp / tb, 4), "ratio_graph": round(tp / tg, 4), "identical_hits": bool(same),
            "passed": int(sum(r["npassed"] for r in batched)), "realigned": int(sum(r["nmap"] for r in batched)),
-           "map_cap_per_query": 100}
+           "map_cap_per_query": 100,
+           "tuning": {"query": mid, "query_len": int(lens[mid]), "seg_cols": seg_cols,
+                      "waves_per_workgroup": nw, "max_rows_per_lane": rmax}}
     log("bench: 64 queries plain %.2f s, per-query launches %.2f s, work queue %.2f s (phase 1 %.1f ms), "
         "queue + graph %.2f s, identical %s" % (tp, tl, tb, qms, tg, same))
     cbp.close()

@@ -623,30 +623,34 @@ extern "C" int cbp_sdp_scores(cbp_handle* h, float* S, void* stream)
 
 /* autotuner: times the phase-1 launch of query qidx over a small grid of
  * (segment columns, waves per workgroup) and keeps the fastest */
-extern "C" int cbp_autotune(cbp_handle* h, int32_t qidx, int32_t* seg_cols, int32_t* nw)
+extern "C" int cbp_autotune(cbp_handle* h, int32_t qidx, int32_t* seg_cols, int32_t* nw, int32_t* rmax)
 {
     if (!h) return CBP_ERR_ARG;
     if (!h->have_opt || !h->have_tables || !h->have_queries || !h->have_chunk)
         return fail(h, CBP_ERR_STATE, "options, tables, queries and chunk must be uploaded first");
+    /* tile width (DB columns per wave segment), waves per workgroup and the
+     * anti-diagonal depth (query rows per lane, at most rmax) */
     static const int segs[] = {2048, 4096, 8192, 16384};
     static const int nws[] = {4, 8};
-    const int rmax = h->rmax;
-    double best = 1e30; int bs = h->seg_cols, bn = h->nw;
-    for (int si = 0; si < 4; si++)
-        for (int ni = 0; ni < 2; ni++) {
-            int rc = cbp_set_tuning(h, segs[si], rmax, nws[ni]);
-            if (rc) return rc;
-            double t = 1e30;
-            for (int rep = 0; rep < 2; rep++) {
-                if ((rc = c2_run_phase1(h, qidx))) return rc;
-                if (h->last_ms < t) t = h->last_ms;
+    static const int rms[] = {5, 4, 3};
+    double best = 1e30; int bs = h->seg_cols, bn = h->nw, br = h->rmax;
+    for (int ri = 0; ri < 3; ri++)
+        for (int si = 0; si < 4; si++)
+            for (int ni = 0; ni < 2; ni++) {
+                int rc = cbp_set_tuning(h, segs[si], rms[ri], nws[ni]);
+                if (rc) return rc;
+                double t = 1e30;
+                for (int rep = 0; rep < 2; rep++) {
+                    if ((rc = c2_run_phase1(h, qidx))) return rc;
+                    if (h->last_ms < t) t = h->last_ms;
+                }
+                if (t < best) { best = t; bs = segs[si]; bn = nws[ni]; br = rms[ri]; }
             }
-            if (t < best) { best = t; bs = segs[si]; bn = nws[ni]; }
-        }
-    int rc = cbp_set_tuning(h, bs, rmax, bn);
+    int rc = cbp_set_tuning(h, bs, br, bn);
     if (rc) return rc;
     if (seg_cols) *seg_cols = bs;
     if (nw) *nw = bn;
+    if (rmax) *rmax = br;
     return CBP_OK;
 }
 

@@ -53,7 +53,7 @@ def lib():
     L.cbp_set_graph.argtypes = [H, C.c_int32]
     L.cbp_set_queue.argtypes = [H, C.c_int32]
     L.cbp_chunk_limits.argtypes = [H, C.c_uint64] + [C.POINTER(C.c_uint64)] * 4
-    L.cbp_autotune.argtypes = [H, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
+    L.cbp_autotune.argtypes = [H, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.cbp_process.argtypes = [H, C.c_int32, C.c_float, C.c_float, C.POINTER(Results)]
     L.cbp_process_batch.argtypes = [H, C.c_void_p, C.c_int32, C.POINTER(Results)]
     L.cbp_score_block.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
@@ -208,11 +208,11 @@ class CuBatchProcessing:
         self._check(self.L.cbp_set_queue(self.h, 1 if on else 0))
 
     def autotune(self, qidx=0):
-        """Times phase 1 of query qidx over a small grid of segment widths and
-        waves per workgroup; keeps and returns the fastest (seg_cols, nw)."""
-        sc, nw = C.c_int32(), C.c_int32()
-        self._check(self.L.cbp_autotune(self.h, qidx, C.byref(sc), C.byref(nw)))
-        return sc.value, nw.value
+        """Times phase 1 of query qidx over segment widths, waves per workgroup
+        and rows per lane; keeps and returns the fastest (seg_cols, nw, rmax)."""
+        sc, nw, rm = C.c_int32(), C.c_int32(), C.c_int32()
+        self._check(self.L.cbp_autotune(self.h, qidx, C.byref(sc), C.byref(nw), C.byref(rm)))
+        return sc.value, nw.value, rm.value
 
 
 def results_to_dict(r):

@@ -170,9 +170,11 @@ int  cbp_chunk_limits(cbp_handle* h, uint64_t qlen, uint64_t* maxpos, uint64_t* 
                       uint64_t* maxpos2, uint64_t* maxpros2);
 
 /* autotuner: times the phase-1 launch of query qidx over segment widths
- * {2048, 4096, 8192, 16384} columns and {4, 8} waves per workgroup, keeps the
- * fastest (cbp_set_tuning) and returns it; no reference counterpart */
-int  cbp_autotune(cbp_handle* h, int32_t qidx, int32_t* seg_cols, int32_t* nw);
+ * {2048, 4096, 8192, 16384} columns (tile width), {4, 8} waves per workgroup
+ * and at most {5, 4, 3} query rows per lane (anti-diagonal depth), keeps the
+ * fastest (cbp_set_tuning) and returns it (any out pointer may be NULL); no
+ * reference counterpart */
+int  cbp_autotune(cbp_handle* h, int32_t qidx, int32_t* seg_cols, int32_t* nw, int32_t* rmax);
 
 /* capture the phase-1 stage of cbp_process / cbp_process_batch in a HIP graph
  * and replay it while the batch, chunk and tuning are unchanged (the first

@@ -139,8 +139,8 @@ def test_autotune_keeps_results():
     cbp.TransferCPMData(d)
     cbp.SetDbDetails(d.npos, d.nprof)
     before = cbp.phase1_keys(0)
-    sc, nw = cbp.autotune(0)
-    assert sc in (2048, 4096, 8192, 16384) and nw in (4, 8)
+    sc, nw, rm = cbp.autotune(0)
+    assert sc in (2048, 4096, 8192, 16384) and nw in (4, 8) and rm in (3, 4, 5)
     assert np.array_equal(cbp.phase1_keys(0), before)
     assert np.array_equal(cbp.phase1_keys(1), cbp.phase1_keys(1))
 
