@@ -633,21 +633,32 @@ extern "C" int cbp_autotune(cbp_handle* h, int32_t qidx, int32_t* seg_cols, int3
     static const int segs[] = {2048, 4096, 8192, 16384};
     static const int nws[] = {4, 8};
     static const int rms[] = {5, 4, 3};
-    double best = 1e30; int bs = h->seg_cols, bn = h->nw, br = h->rmax;
+    /* the current tuning is the baseline; another geometry replaces it only
+     * when faster by more than 1% (geometries that are the same launch for
+     * this query, or within timing noise, keep the baseline, which the other
+     * query lengths of a batch were tuned for) */
+    auto time_it = [&](int sc, int rm, int nwv, double* t) {
+        int rc = cbp_set_tuning(h, sc, rm, nwv);
+        if (rc) return rc;
+        *t = 1e30;
+        for (int rep = 0; rep < 2; rep++) {
+            if ((rc = c2_run_phase1(h, qidx))) return rc;
+            if (h->last_ms < *t) *t = h->last_ms;
+        }
+        return CBP_OK;
+    };
+    int bs = h->seg_cols, bn = h->nw, br = h->rmax;
+    double best = 1e30;
+    int rc = time_it(bs, br, bn, &best);
+    if (rc) return rc;
     for (int ri = 0; ri < 3; ri++)
         for (int si = 0; si < 4; si++)
             for (int ni = 0; ni < 2; ni++) {
-                int rc = cbp_set_tuning(h, segs[si], rms[ri], nws[ni]);
-                if (rc) return rc;
                 double t = 1e30;
-                for (int rep = 0; rep < 2; rep++) {
-                    if ((rc = c2_run_phase1(h, qidx))) return rc;
-                    if (h->last_ms < t) t = h->last_ms;
-                }
-                if (t < best) { best = t; bs = segs[si]; bn = nws[ni]; br = rms[ri]; }
+                if ((rc = time_it(segs[si], rms[ri], nws[ni], &t))) return rc;
+                if (t < 0.99 * best) { best = t; bs = segs[si]; bn = nws[ni]; br = rms[ri]; }
             }
-    int rc = cbp_set_tuning(h, bs, br, bn);
-    if (rc) return rc;
+    if ((rc = cbp_set_tuning(h, bs, br, bn))) return rc;
     if (seg_cols) *seg_cols = bs;
     if (nw) *nw = bn;
     if (rmax) *rmax = br;

@@ -8,6 +8,7 @@
 #include <vector>
 #include <memory>
 #include <map>
+#include <new>
 #include "../../include/comer2_hip.h"
 #include "c2_prep.h"
 #include "c2_kernels.h"
@@ -18,6 +19,27 @@ struct DBuf {
     ~DBuf() { if (p) (void)hipFree(p); }
     template <class T> T* as() const { return (T*)p; }
 };
+
+/* host vectors in pinned memory: the stage buffers that cross to and from
+ * the device, so that their copies are asynchronous and the queries of a
+ * batch overlap on their streams (a copy to or from pageable memory waits
+ * for the stream); allocation failure throws std::bad_alloc, caught at the
+ * C ABI */
+template <class T>
+struct PinAlloc {
+    using value_type = T;
+    PinAlloc() = default;
+    template <class U> PinAlloc(const PinAlloc<U>&) {}
+    T* allocate(size_t n) {
+        void* p = nullptr;
+        if (hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault) != hipSuccess) throw std::bad_alloc();
+        return static_cast<T*>(p);
+    }
+    void deallocate(T* p, size_t) { (void)hipHostFree(p); }
+    template <class U> bool operator==(const PinAlloc<U>&) const { return true; }
+    template <class U> bool operator!=(const PinAlloc<U>&) const { return false; }
+};
+template <class T> using pvec = std::vector<T, PinAlloc<T>>;
 
 /* owned host copy of a profile set */
 struct HostSet {
@@ -51,8 +73,8 @@ struct QWork {
     DBuf keys;                          /* [nprof] phase-1 maxima (one-query runs) */
     unsigned long long* hkeys = nullptr; /* pinned host copy of keys (one-query runs) */
     size_t hkeys_n = 0;
-    std::vector<int32_t> cand, pe;
-    std::vector<int64_t> off;
+    pvec<int32_t> cand, pe;
+    pvec<int64_t> off;
     DBuf wts;                           /* [nprof][20] score weights of the query (one-query runs) */
     /* the buffers the stages use: the query's own above, or its slice of a
      * batch's work-queue buffers */
@@ -60,19 +82,19 @@ struct QWork {
     unsigned long long* hk = nullptr;   /* pinned host copy [nprof] */
     float* dw = nullptr;                /* device score weights [nprof][20] */
     DBuf dcand, dpe, doff, btck, fin;
-    std::vector<FinRec> hfin;
-    std::vector<int32_t> pass;
+    pvec<FinRec> hfin;
+    pvec<int32_t> pass;
     std::vector<int> passfin;
-    std::vector<StatIn> si;
+    pvec<StatIn> si;
     DBuf pass_pro, counts, statin, statout;
-    std::vector<double> hstat;
+    pvec<double> hstat;
     std::vector<int> sel;
-    std::vector<MapPair> mp;
-    std::vector<int32_t> morder;
-    std::vector<int4> msitems;          /* dynamic-score work items */
+    pvec<MapPair> mp;
+    pvec<int32_t> morder;
+    pvec<int4> msitems;                 /* dynamic-score work items */
     int64_t alnstride = 0;
     DBuf mpairs, dmorder, dmsitems, mF, mSd, mhand, mbt, mstat, malns, malntmp, mres;
-    std::vector<int32_t> hmres;
+    pvec<int32_t> hmres;
     int64_t nposits = 0;
     int32_t maxlen = 0;
     uint64_t sspace = 0;

@@ -750,10 +750,21 @@ int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* 
     return CBP_OK;
 }
 
+/* c2_run_queries behind the C ABI: no exception crosses it */
+static int run_queries_guarded(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* scorethld,
+                               const float* logevthld, cbp_results* outs)
+{
+    try {
+        return c2_run_queries(h, qidx, nq, scorethld, logevthld, outs);
+    } catch (const std::bad_alloc&) {
+        return c2_fail(h, CBP_ERR_MEM, "host memory allocation failed");
+    }
+}
+
 extern "C" int cbp_process(cbp_handle* h, int32_t qidx, float scorethld, float logevthld, cbp_results* out)
 {
     if (!h || !out) return CBP_ERR_ARG;
-    return c2_run_queries(h, &qidx, 1, &scorethld, &logevthld, out);
+    return run_queries_guarded(h, &qidx, 1, &scorethld, &logevthld, out);
 }
 
 extern "C" int cbp_process_batch(cbp_handle* h, const int32_t* qidx, int32_t nq, cbp_results* outs)
@@ -769,7 +780,7 @@ extern "C" int cbp_process_batch(cbp_handle* h, const int32_t* qidx, int32_t nq,
         const int rc = cbp_score_threshold(&h->opt, (uint64_t)h->queries.v.len[qidx[i]], dblen, &st[i], &lt[i]);
         if (rc) return c2_fail(h, rc, "score threshold");
     }
-    return c2_run_queries(h, qidx, nq, st.data(), lt.data(), outs);
+    return run_queries_guarded(h, qidx, nq, st.data(), lt.data(), outs);
 }
 
 extern "C" int cbp_set_graph(cbp_handle* h, int32_t on)
