@@ -19,14 +19,21 @@ __device__ __forceinline__ S5 s5_zero() { S5 z; z.mm = z.mi = z.im = z.dn = z.nd
  * (cudp/CuBatchDP_init_btck.cu:303-360).  Terms that share the same DB
  * transition are maximised before that transition is added; because FP32
  * rounding is monotonic, max(fl(a+d), fl(b+d)) == fl(max(a,b)+d), so the
- * result is bit-identical to the reference's term-by-term order. */
+ * result is bit-identical to the reference's term-by-term order.
+ * dp_mm is the MM state alone, the only state that reads the diagonal. */
+__device__ __forceinline__ float dp_mm(const S5& dg, const float* qt, const float* dt, float s)
+{
+    float m0 = fmaxf(fmaxf(dg.im + qt[3], dg.dn + qt[5]), dg.mm + qt[0]) + dt[0];
+    float v1 = fmaxf(fmaxf((dg.mi + qt[0]) + dt[3], (dg.nd + qt[0]) + dt[5]), m0);
+    return fmaxf(v1 + s, 0.0f);
+}
+
+/* the whole cell */
 __device__ __forceinline__ S5 dp_cell(const S5& dg, const S5& up, const S5& lf,
                                       const float* qt, const float* dt, float s)
 {
     S5 n;
-    float m0 = fmaxf(fmaxf(dg.im + qt[3], dg.dn + qt[5]), dg.mm + qt[0]) + dt[0];
-    float v1 = fmaxf(fmaxf((dg.mi + qt[0]) + dt[3], (dg.nd + qt[0]) + dt[5]), m0);
-    n.mm = fmaxf(v1 + s, 0.0f);
+    n.mm = dp_mm(dg, qt, dt, s);
     n.mi = fmaxf((up.mm + qt[0]) + dt[1], (up.mi + qt[0]) + dt[4]);
     n.im = fmaxf(lf.mm + qt[1], lf.im + qt[4]) + dt[0];
     n.dn = fmaxf(up.mm + qt[2], up.dn + qt[6]);
@@ -52,6 +59,14 @@ __device__ __forceinline__ float wave_shr1(float v, float old)
 {
     return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
         __builtin_bit_cast(int, old), __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, false));
+}
+
+/* wave_shr1 with 0 into lane 0 (DPP bound_ctrl: the missing source reads
+ * as zero, so no register has to hold the old value) */
+__device__ __forceinline__ float wave_shr1_z(float v)
+{
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(
+        0, __builtin_bit_cast(int, v), 0x138, 0xf, 0xf, true));
 }
 
 /* whole-wave shift by one lane towards lower lanes (DPP wave_shl:1);

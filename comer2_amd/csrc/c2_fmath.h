@@ -52,22 +52,32 @@ C2_HD float c2_u2f(uint32_t u) {
 #define C2_FLT_MIN_NORMAL 1.17549435e-38f
 
 /* natural logarithm for x >= FLT_MIN (callers clamp; see c2_logf_clamped) */
+#define C2_LOG_P7 0.08516032248735428f
+#define C2_LOG_P6 (-0.14337590336799622f)
+#define C2_LOG_P5 0.15015284717082977f
+#define C2_LOG_P4 (-0.16570599377155304f)
+#define C2_LOG_P3 0.19951635599136353f
+#define C2_LOG_P2 (-0.25001534819602966f)
+#define C2_LOG_P1 0.33334264159202576f
+#define C2_LOG_P0 (-0.4999999403953552f)
+#define C2_LN2_HI 0.693145751953125f
+#define C2_LN2_LO 1.428606765330187e-06f
 C2_HD float c2_logf(float x) {
     uint32_t ix = c2_f2u(x) - 0x3f3504f3u;          /* 0x3f3504f3 = sqrt(1/2) */
     int32_t k = ((int32_t)ix) >> 23;
     uint32_t iy = (ix & 0x007fffffu) + 0x3f3504f3u;
     float f = c2_u2f(iy) - 1.0f;                     /* exact */
-    float p = 0.08516032248735428f;
-    p = fmaf(p, f, -0.14337590336799622f);
-    p = fmaf(p, f, 0.15015284717082977f);
-    p = fmaf(p, f, -0.16570599377155304f);
-    p = fmaf(p, f, 0.19951635599136353f);
-    p = fmaf(p, f, -0.25001534819602966f);
-    p = fmaf(p, f, 0.33334264159202576f);
-    p = fmaf(p, f, -0.4999999403953552f);
+    float p = C2_LOG_P7;
+    p = fmaf(p, f, C2_LOG_P6);
+    p = fmaf(p, f, C2_LOG_P5);
+    p = fmaf(p, f, C2_LOG_P4);
+    p = fmaf(p, f, C2_LOG_P3);
+    p = fmaf(p, f, C2_LOG_P2);
+    p = fmaf(p, f, C2_LOG_P1);
+    p = fmaf(p, f, C2_LOG_P0);
     float r = fmaf(f * f, p, f);
     float kf = (float)k;
-    return fmaf(kf, 0.693145751953125f, fmaf(kf, 1.428606765330187e-06f, r));
+    return fmaf(kf, C2_LN2_HI, fmaf(kf, C2_LN2_LO, r));
 }
 
 /* arguments are sums of non-negative products (never NaN): fmaxf gives the
@@ -75,6 +85,41 @@ C2_HD float c2_logf(float x) {
 C2_HD float c2_logf_clamped(float x) {
     return c2_logf(fmaxf(x, C2_FLT_MIN_NORMAL));
 }
+
+#if defined(__HIPCC__)
+/* c2_logf_clamped of N values, the same operations step by step across the
+ * values, so that the N independent chains interleave in one wave */
+template <int N>
+__device__ __forceinline__ void c2_logf_clamped_n(const float* x, float* out) {
+    float f[N], p[N], kf[N];
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        uint32_t ix = c2_f2u(fmaxf(x[i], C2_FLT_MIN_NORMAL)) - 0x3f3504f3u;
+        kf[i] = (float)(((int32_t)ix) >> 23);
+        f[i] = c2_u2f((ix & 0x007fffffu) + 0x3f3504f3u) - 1.0f;
+        p[i] = C2_LOG_P7;
+    }
+#pragma unroll
+    for (int i = 0; i < N; i++) p[i] = fmaf(p[i], f[i], C2_LOG_P6);
+#pragma unroll
+    for (int i = 0; i < N; i++) p[i] = fmaf(p[i], f[i], C2_LOG_P5);
+#pragma unroll
+    for (int i = 0; i < N; i++) p[i] = fmaf(p[i], f[i], C2_LOG_P4);
+#pragma unroll
+    for (int i = 0; i < N; i++) p[i] = fmaf(p[i], f[i], C2_LOG_P3);
+#pragma unroll
+    for (int i = 0; i < N; i++) p[i] = fmaf(p[i], f[i], C2_LOG_P2);
+#pragma unroll
+    for (int i = 0; i < N; i++) p[i] = fmaf(p[i], f[i], C2_LOG_P1);
+#pragma unroll
+    for (int i = 0; i < N; i++) p[i] = fmaf(p[i], f[i], C2_LOG_P0);
+#pragma unroll
+    for (int i = 0; i < N; i++) {
+        const float r = fmaf(f[i] * f[i], p[i], f[i]);
+        out[i] = fmaf(kf[i], C2_LN2_HI, fmaf(kf[i], C2_LN2_LO, r));
+    }
+}
+#endif
 
 /* exponential for x in [-87.3, 88.7]; callers clamp or threshold */
 C2_HD float c2_expf(float x) {

@@ -36,9 +36,12 @@ namespace {
 
 constexpr int RING = 96;      /* columns in the per-wave LDS ring */
 constexpr int BLK = 32;       /* columns loaded per refill */
-/* ring field offsets (each field is RING words) */
+/* the ring is slot-major: column slot s holds its fields at words
+ * [s*NS, s*NS + NS), so a lane reads all of its column's fields at immediate
+ * offsets from one address; NS is odd, so the 64 lanes (64 consecutive slots)
+ * hit 64 distinct banks.  Field offsets within a slot: */
 constexpr int F_DW = 0, F_DT = 20, F_PJ = 27, F_X = 30, F_LEN = 31, F_PID = 32, F_BIN = 33, F_TOP = 34;
-constexpr int NF_BASE = 34, NF_TOP = 39;
+constexpr int NS_BASE = 35, NS_TOP = 39;   /* slot stride without / with the stripe-top fields */
 
 /* tops: the ring carries the stripe's top boundary; rtop: read it from the
  * hand-off buffer (stripes after the first) instead of the zero row -1 */
@@ -52,12 +55,12 @@ __device__ __forceinline__ int block_profile(PA& a, int lane, int t0, int64_t c_
 }
 
 template <int R, class PA>
-__device__ __forceinline__ void load_block(PA& a, float* ring, int lane, int t0,
+__device__ __forceinline__ void load_block(PA& a, float* ring, int ns, int lane, int t0,
                                            int64_t c_begin, int64_t c_end, bool tops, bool rtop, int p,
                                            const float* bot, int64_t bld, int64_t bc0)
 {
     if (lane >= BLK) return;
-    const int slot = (t0 + lane) % RING;
+    float* rc = ring + ((t0 + lane) % RING) * ns;
     const int64_t c = c_begin + t0 + lane;
     if (c < c_end) {
         const int x = (int)(c - (int64_t)a.dist[p]);
@@ -72,48 +75,48 @@ __device__ __forceinline__ void load_block(PA& a, float* ring, int lane, int t0,
              * are formed here (the same FP32 operations as the table) */
             for (int k = 0; k < 20; k++) {
                 const float w = 1.0f / (0.5f * (a.qbkg[k] + a.bkg[(size_t)k * a.nprof + p]));
-                ring[(F_DW + k) * RING + slot] = a.trgfrq[(size_t)k * a.npos + c] * w;
+                rc[F_DW + k] = a.trgfrq[(size_t)k * a.npos + c] * w;
             }
         } else {
             const float* wp = a.wts + (size_t)p * 20;
             for (int k = 0; k < 20; k++)
-                ring[(F_DW + k) * RING + slot] = tf[(size_t)k * npos] * wp[k];
+                rc[F_DW + k] = tf[(size_t)k * npos] * wp[k];
         }
         size_t ntr = (size_t)(a.npos + a.nprof);
         asm volatile("" : "+s"(ntr));
         const float* tr = a.trans + (size_t)c + p;     /* row x of profile p: dist + p + x */
-        ring[(F_DT + 0) * RING + slot] = tr[0 * ntr];
-        ring[(F_DT + 1) * RING + slot] = tr[1 * ntr + 1];
-        ring[(F_DT + 2) * RING + slot] = tr[2 * ntr];
-        ring[(F_DT + 3) * RING + slot] = tr[3 * ntr];
-        ring[(F_DT + 4) * RING + slot] = tr[4 * ntr + 1];
-        ring[(F_DT + 5) * RING + slot] = tr[5 * ntr];
-        ring[(F_DT + 6) * RING + slot] = tr[6 * ntr];
+        rc[F_DT + 0] = tr[0 * ntr];
+        rc[F_DT + 1] = tr[1 * ntr + 1];
+        rc[F_DT + 2] = tr[2 * ntr];
+        rc[F_DT + 3] = tr[3 * ntr];
+        rc[F_DT + 4] = tr[4 * ntr + 1];
+        rc[F_DT + 5] = tr[5 * ntr];
+        rc[F_DT + 6] = tr[6 * ntr];
         const float* sp = a.ssprb + c;
         for (int j = 0; j < 3; j++)
-            ring[(F_PJ + j) * RING + slot] = sp[(size_t)j * npos];
-        ring[F_X * RING + slot] = __int_as_float(x);
-        ring[F_LEN * RING + slot] = __int_as_float(a.len[p]);
-        ring[F_PID * RING + slot] = __int_as_float(p);
+            rc[F_PJ + j] = sp[(size_t)j * npos];
+        rc[F_X] = __int_as_float(x);
+        rc[F_LEN] = __int_as_float(a.len[p]);
+        rc[F_PID] = __int_as_float(p);
         /* the column's SS-table bins int(9.999 p'_j), 4 bits each */
         const uint32_t bins = (uint32_t)(int)(sp[0] * 9.999f) | ((uint32_t)(int)(sp[(size_t)npos] * 9.999f) << 4) |
                               ((uint32_t)(int)(sp[2 * (size_t)npos] * 9.999f) << 8);
-        ring[F_BIN * RING + slot] = __uint_as_float(bins);
+        rc[F_BIN] = __uint_as_float(bins);
         if (tops) {
             const float* bt = bot + (c - bc0);
             for (int s = 0; s < 5; s++)
-                ring[(F_TOP + s) * RING + slot] = rtop ? bt[(size_t)s * bld] : 0.0f;
+                rc[F_TOP + s] = rtop ? bt[(size_t)s * bld] : 0.0f;
         }
     } else {
-        for (int k = 0; k < 20; k++) ring[(F_DW + k) * RING + slot] = 0.0f;
-        for (int k = 0; k < 7; k++) ring[(F_DT + k) * RING + slot] = -32768.0f;
-        for (int j = 0; j < 3; j++) ring[(F_PJ + j) * RING + slot] = 0.0f;
-        ring[F_X * RING + slot] = __int_as_float(-1);
-        ring[F_LEN * RING + slot] = __int_as_float(1);
-        ring[F_PID * RING + slot] = __int_as_float(-1);
-        ring[F_BIN * RING + slot] = 0.0f;
+        for (int k = 0; k < 20; k++) rc[F_DW + k] = 0.0f;
+        for (int k = 0; k < 7; k++) rc[F_DT + k] = -32768.0f;
+        for (int j = 0; j < 3; j++) rc[F_PJ + j] = 0.0f;
+        rc[F_X] = __int_as_float(-1);
+        rc[F_LEN] = __int_as_float(1);
+        rc[F_PID] = __int_as_float(-1);
+        rc[F_BIN] = 0.0f;
         if (tops)
-            for (int s = 0; s < 5; s++) ring[(F_TOP + s) * RING + slot] = 0.0f;
+            for (int s = 0; s < 5; s++) rc[F_TOP + s] = 0.0f;
     }
 }
 
@@ -140,9 +143,9 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
     float* sLL = sQT + 30 * SL;                 /* [5][SL] */
     float* sLD = sLL + 5 * SL;                  /* [5][SL] */
     const bool tops = a.nstripes > 1;
-    const int nf = tops ? NF_TOP : NF_BASE;
+    const int ns = tops ? NS_TOP : NS_BASE;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-    float* ring = sLD + 5 * SL + (size_t)wv * nf * RING;
+    float* ring = sLD + 5 * SL + (size_t)wv * ns * RING;
 
     for (int s = 0; s < a.nstripes; s++) {
         const int rbase = s * SL;
@@ -192,18 +195,19 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
 
         /* ring: invalid columns everywhere, then the first block */
         for (int i = lane; i < RING; i += 64) {
-            for (int k = 0; k < 20; k++) ring[(F_DW + k) * RING + i] = 0.0f;
-            for (int k = 0; k < 7; k++) ring[(F_DT + k) * RING + i] = -32768.0f;
-            for (int j = 0; j < 3; j++) ring[(F_PJ + j) * RING + i] = 0.0f;
-            ring[F_X * RING + i] = __int_as_float(-1);
-            ring[F_LEN * RING + i] = __int_as_float(1);
-            ring[F_PID * RING + i] = __int_as_float(-1);
-            ring[F_BIN * RING + i] = 0.0f;
-            if (tops) for (int j = 0; j < 5; j++) ring[(F_TOP + j) * RING + i] = 0.0f;
+            float* rc = ring + i * ns;
+            for (int k = 0; k < 20; k++) rc[F_DW + k] = 0.0f;
+            for (int k = 0; k < 7; k++) rc[F_DT + k] = -32768.0f;
+            for (int j = 0; j < 3; j++) rc[F_PJ + j] = 0.0f;
+            rc[F_X] = __int_as_float(-1);
+            rc[F_LEN] = __int_as_float(1);
+            rc[F_PID] = __int_as_float(-1);
+            rc[F_BIN] = 0.0f;
+            if (tops) for (int j = 0; j < 5; j++) rc[F_TOP + j] = 0.0f;
         }
         wave_lds_sync();
         int pn = block_profile(a, lane, 0, c_begin, c_end);
-        load_block<R>(a, ring, lane, 0, c_begin, c_end, tops, s > 0, pn, bot, bld, bc0);
+        load_block<R>(a, ring, ns, lane, 0, c_begin, c_end, tops, s > 0, pn, bot, bld, bc0);
         wave_lds_sync();
 
         S5 H[R];
@@ -211,9 +215,9 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
         for (int k = 0; k < R; k++) H[k] = s5_zero();
         S5 U = s5_zero(), D = s5_zero();
         if (tops && lane == 0) {
-            U.mm = ring[(F_TOP + 0) * RING]; U.mi = ring[(F_TOP + 1) * RING];
-            U.im = ring[(F_TOP + 2) * RING]; U.dn = ring[(F_TOP + 3) * RING];
-            U.nd = ring[(F_TOP + 4) * RING];
+            U.mm = ring[F_TOP + 0]; U.mi = ring[F_TOP + 1];
+            U.im = ring[F_TOP + 2]; U.dn = ring[F_TOP + 3];
+            U.nd = ring[F_TOP + 4];
         }
         float best = 0.0f;
         int bpos = 0, curpid = -1, pstart = 0, cA = 0, cW = 0;
@@ -221,22 +225,31 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
         uint32_t* btw = nullptr;
         if (BTCK) btw = a.btck + a.btck_off[seg] + (size_t)s * bstride * 64;
 
+        /* word offset of the lane's column slot, ((t - lane) mod RING) * ns,
+         * advanced by one slot per step */
+        const int soff_end = (RING - 1) * ns;
+        int soff = ((RING * 4 - lane) % RING) * ns;
         for (int t = 0; t < nsteps; t++) {
             if (((t + 1) % BLK) == 0) {
-                load_block<R>(a, ring, lane, t + 1, c_begin, c_end, tops, s > 0,
+                load_block<R>(a, ring, ns, lane, t + 1, c_begin, c_end, tops, s > 0,
                               block_profile(a, lane, t + 1, c_begin, c_end), bot, bld, bc0);
                 wave_lds_sync();
             }
 
-            const int slot = (t - lane + RING * 4) % RING;
+            /* one LDS address for all of the column's fields: opaque, so that
+             * the paired reads take small offsets from it instead of one
+             * address per pair */
+            const __attribute__((address_space(3))) float* rs =
+                (const __attribute__((address_space(3))) float*)(ring + soff);
+            asm volatile("" : "+v"(rs));
             float dw[20], dt[7], pj[3];
 #pragma unroll
-            for (int k = 0; k < 20; k++) dw[k] = ring[(F_DW + k) * RING + slot];
+            for (int k = 0; k < 20; k++) dw[k] = rs[F_DW + k];
 #pragma unroll
-            for (int k = 0; k < 7; k++) dt[k] = ring[(F_DT + k) * RING + slot];
+            for (int k = 0; k < 7; k++) dt[k] = rs[F_DT + k];
 #pragma unroll
-            for (int j = 0; j < 3; j++) pj[j] = ring[(F_PJ + j) * RING + slot];
-            const int xloc = __float_as_int(ring[F_X * RING + slot]);
+            for (int j = 0; j < 3; j++) pj[j] = rs[F_PJ + j];
+            const int xloc = __float_as_int(rs[F_X]);
 
             /* profile start: flush, reset to the column -1 boundary */
             const bool first = (xloc == 0);
@@ -247,8 +260,8 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
                         const int tb = bpos >> 3, kb = bpos & 7;
                         atomicMax(a.keys + curpid, mkkey(best, (tb - lane) - pstart, y0 + kb));
                     }
-                    const int ldb = __float_as_int(ring[F_LEN * RING + slot]);
-                    curpid = __float_as_int(ring[F_PID * RING + slot]);
+                    const int ldb = __float_as_int(rs[F_LEN]);
+                    curpid = __float_as_int(rs[F_PID]);
                     pstart = t - lane;
                     best = 0.0f; bpos = 0;
                     const int mx = a.Lq > ldb ? a.Lq : ldb;
@@ -275,40 +288,70 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
             /* per-column SS table offsets: QT[10j + int(9.999 p'_j)] */
             int co[3];
             {
-                const uint32_t bb = __float_as_uint(ring[F_BIN * RING + slot]);
+                const uint32_t bb = __float_as_uint(rs[F_BIN]);
 #pragma unroll
                 for (int j = 0; j < 3; j++) co[j] = (10 * j + (int)((bb >> (4 * j)) & 15u)) * SL + R * lane;
             }
             const float dh = (pj[2] + 0.1f) * 0.1111111f;
             const int u0 = (xloc - y0) - cA - 1;
 
+            /* the rows of one step: their scores first, each stage across all
+             * rows (independent chains interleave), then the DP row by row; the
+             * SS term is added with weight 0 when it is off (an exact +-0) */
+            float sc[R];
+            {
+                float acc[R];
+#pragma unroll
+                for (int k = 0; k < R; k++) acc[k] = q[k][0] * dw[0];
+#pragma unroll
+                for (int j = 1; j < 20; j++)
+#pragma unroll
+                    for (int k = 0; k < R; k++) acc[k] = fmaf(q[k][j], dw[j], acc[k]);
+                c2_logf_clamped_n<R>(acc, sc);
+                const float ssw = a.ssswgt > 0.0f ? a.ssswgt : 0.0f;
+                float t0[R], t1[R], t2[R];
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    t0[k] = sQT[co[0] + k]; t1[k] = sQT[co[1] + k]; t2[k] = sQT[co[2] + k];
+                }
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    float ss = t0[k] * pj[0];
+                    ss = fmaf(t1[k], pj[1], ss);
+                    ss = fmaf(t2[k], pj[2], ss);
+                    ss = fmaf(qh[k], dh, ss);
+                    ss = (ss - 0.05f) * ssw;
+                    sc[k] = (sc[k] - a.shift) + ss;
+                    if ((uint32_t)(u0 - k) >= (uint32_t)cW) sc[k] = -9.0f;   /* distant corners */
+                }
+            }
+            /* the stripe top of the next step's column, read ahead of the rows */
+            S5 top;
+            if (tops) {
+                const float* rt = ring + ((t + 1) % RING) * ns;
+                top.mm = rt[F_TOP + 0]; top.mi = rt[F_TOP + 1];
+                top.im = rt[F_TOP + 2]; top.dn = rt[F_TOP + 3];
+                top.nd = rt[F_TOP + 4];
+            }
             S5 saved = D;
             uint32_t bw = 0;
-#pragma unroll
-            for (int k = 0; k < R; k++) {
-                /* score: log(sum_a q_a d_a w_a) - shift + SS term */
-                float acc = q[k][0] * dw[0];
-#pragma unroll
-                for (int j = 1; j < 20; j++) acc = fmaf(q[k][j], dw[j], acc);
-                float sc = c2_logf_clamped(acc) - a.shift;
-                if (a.ssswgt > 0.0f) {
-                    float ss = sQT[co[0] + k] * pj[0];
-                    ss = fmaf(sQT[co[1] + k], pj[1], ss);
-                    ss = fmaf(sQT[co[2] + k], pj[2], ss);
-                    ss = fmaf(qh[k], dh, ss);
-                    ss = (ss - 0.05f) * a.ssswgt;
-                    sc = sc + ss;
-                }
-                if ((uint32_t)(u0 - k) >= (uint32_t)cW) sc = -9.0f;   /* distant corners */
-                S5 dg = saved;
-                if (k > 0 && (fixbits & (1u << k))) {
-                    if (first && kf == k) dg = s5_zero();
-                }
+            auto row = [&](const int k, const bool fix) {
+                const S5 dg = saved;
                 const S5 up = (k == 0) ? U : H[k - 1];
                 saved = H[k];
-                const S5 n = dp_cell(dg, up, H[k], qt[k], dt, sc);
+                S5 n = dp_cell(dg, up, H[k], qt[k], dt, sc[k]);
+                /* row kf at a profile's first column: its diagonal reads zeros */
+                if (fix && k > 0 && (fixbits & (1u << k)) && first && kf == k)
+                    n.mm = dp_mm(s5_zero(), qt[k], dt, sc[k]);
                 H[k] = n;
                 if (BTCK) bw |= dp_btck(n) << (2 * k);
+            };
+            if (fixbits == 0) {
+#pragma unroll
+                for (int k = 0; k < R; k++) row(k, false);
+            } else {
+#pragma unroll
+                for (int k = 0; k < R; k++) row(k, true);
             }
             /* running maximum: the step's largest cell, the first row on ties
              * (the order of a row-by-row strict-greater update) */
@@ -325,29 +368,31 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
                 }
             }
             if (BTCK) btw[(size_t)t * 64 + lane] = bw;
-            if (wbot && lane == 63) {
-                const int64_t c = c_begin + t - 63;
-                if (c >= c_begin && c < c_end) {
-                    float* bt = bot + (c - bc0);
+            if (wbot && t >= 63 && t - 63 < ncols) {
+                if (lane == 63) {
+                    float* bt = bot + ((c_begin + t - 63) - bc0);
                     bt[0 * (size_t)bld] = H[R - 1].mm; bt[1 * (size_t)bld] = H[R - 1].mi;
                     bt[2 * (size_t)bld] = H[R - 1].im; bt[3 * (size_t)bld] = H[R - 1].dn;
                     bt[4 * (size_t)bld] = H[R - 1].nd;
                 }
             }
-            /* hand the last row to the next lane; lane 0 takes the stripe's top boundary */
-            S5 top = s5_zero();
-            if (tops) {
-                const int ns = (t + 1) % RING;
-                top.mm = ring[(F_TOP + 0) * RING + ns]; top.mi = ring[(F_TOP + 1) * RING + ns];
-                top.im = ring[(F_TOP + 2) * RING + ns]; top.dn = ring[(F_TOP + 3) * RING + ns];
-                top.nd = ring[(F_TOP + 4) * RING + ns];
-            }
+            /* hand the last row to the next lane; lane 0 takes the stripe's
+             * top boundary (zeros in the first stripe) */
             D = U;
-            U.mm = wave_shr1(H[R - 1].mm, top.mm);
-            U.mi = wave_shr1(H[R - 1].mi, top.mi);
-            U.im = wave_shr1(H[R - 1].im, top.im);
-            U.dn = wave_shr1(H[R - 1].dn, top.dn);
-            U.nd = wave_shr1(H[R - 1].nd, top.nd);
+            if (tops) {
+                U.mm = wave_shr1(H[R - 1].mm, top.mm);
+                U.mi = wave_shr1(H[R - 1].mi, top.mi);
+                U.im = wave_shr1(H[R - 1].im, top.im);
+                U.dn = wave_shr1(H[R - 1].dn, top.dn);
+                U.nd = wave_shr1(H[R - 1].nd, top.nd);
+            } else {
+                U.mm = wave_shr1_z(H[R - 1].mm);
+                U.mi = wave_shr1_z(H[R - 1].mi);
+                U.im = wave_shr1_z(H[R - 1].im);
+                U.dn = wave_shr1_z(H[R - 1].dn);
+                U.nd = wave_shr1_z(H[R - 1].nd);
+            }
+            soff = (soff == soff_end) ? 0 : soff + ns;
         }
         if (curpid >= 0 && best > 0.0f) {
             const int tb = bpos >> 3, kb = bpos & 7;
@@ -444,8 +489,8 @@ hipError_t c2_launch_zero_u32(uint32_t* p, int64_t n, hipStream_t stream)
 
 size_t c2_phase1_lds_bytes(int R, int NW, int nstripes)
 {
-    const int nf = nstripes > 1 ? NF_TOP : NF_BASE;
-    return sizeof(float) * ((size_t)40 * 64 * R + (size_t)NW * nf * RING);
+    const int ns = nstripes > 1 ? NS_TOP : NS_BASE;
+    return sizeof(float) * ((size_t)40 * 64 * R + (size_t)NW * ns * RING);
 }
 
 #define C2_P1_CASE(RR, NWW, BT)                                                         \
