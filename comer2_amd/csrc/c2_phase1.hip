@@ -229,27 +229,31 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
          * advanced by one slot per step */
         const int soff_end = (RING - 1) * ns;
         int soff = ((RING * 4 - lane) % RING) * ns;
-        for (int t = 0; t < nsteps; t++) {
-            if (((t + 1) % BLK) == 0) {
-                load_block<R>(a, ring, ns, lane, t + 1, c_begin, c_end, tops, s > 0,
-                              block_profile(a, lane, t + 1, c_begin, c_end), bot, bld, bc0);
-                wave_lds_sync();
-            }
-
+        /* a step's column fields are read at the end of the step before, after
+         * that step's refill: the ring then holds the 64 columns the lanes read
+         * next and the 32 columns of the block after them */
+        float dw[20], dt[7], pj[3];
+        int xloc;
+        uint32_t bb;
+        auto read_col = [&]() {
             /* one LDS address for all of the column's fields: opaque, so that
              * the paired reads take small offsets from it instead of one
              * address per pair */
             const __attribute__((address_space(3))) float* rs =
                 (const __attribute__((address_space(3))) float*)(ring + soff);
             asm volatile("" : "+v"(rs));
-            float dw[20], dt[7], pj[3];
 #pragma unroll
             for (int k = 0; k < 20; k++) dw[k] = rs[F_DW + k];
 #pragma unroll
             for (int k = 0; k < 7; k++) dt[k] = rs[F_DT + k];
 #pragma unroll
             for (int j = 0; j < 3; j++) pj[j] = rs[F_PJ + j];
-            const int xloc = __float_as_int(rs[F_X]);
+            xloc = __float_as_int(rs[F_X]);
+            bb = __float_as_uint(rs[F_BIN]);
+        };
+        read_col();
+        for (int t = 0; t < nsteps; t++) {
+            const float* rs = ring + soff;
 
             /* profile start: flush, reset to the column -1 boundary */
             const bool first = (xloc == 0);
@@ -288,7 +292,6 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
             /* per-column SS table offsets: QT[10j + int(9.999 p'_j)] */
             int co[3];
             {
-                const uint32_t bb = __float_as_uint(rs[F_BIN]);
 #pragma unroll
                 for (int j = 0; j < 3; j++) co[j] = (10 * j + (int)((bb >> (4 * j)) & 15u)) * SL + R * lane;
             }
@@ -393,6 +396,14 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
                 U.nd = wave_shr1_z(H[R - 1].nd);
             }
             soff = (soff == soff_end) ? 0 : soff + ns;
+            /* the block after next: its first column is the top boundary the
+             * next step reads ahead of its rows */
+            if (((t + 2) % BLK) == 0) {
+                load_block<R>(a, ring, ns, lane, t + 2, c_begin, c_end, tops, s > 0,
+                              block_profile(a, lane, t + 2, c_begin, c_end), bot, bld, bc0);
+                wave_lds_sync();
+            }
+            read_col();
         }
         if (curpid >= 0 && best > 0.0f) {
             const int tb = bpos >> 3, kb = bpos & 7;
