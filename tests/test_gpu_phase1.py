@@ -51,3 +51,23 @@ def test_phase1_keys_edge_lengths(rmax):
         assert bad.size == 0, "Lq %d: profiles %s (lengths %s) got %s exp %s" % (
             EDGE_Q[qp], bad[:5], [EDGE_D[i] for i in bad[:5]],
             [hex(int(got[i])) for i in bad[:3]], [hex(int(exp[i])) for i in bad[:3]])
+
+
+@pytest.mark.parametrize("rmax", [5, 3])
+def test_phase1_keys_ss_term_off(rmax):
+    """ssswgt = 0: the kernel adds the SS term with weight 0 (an exact +-0),
+    the oracle skips it; the keys must agree bit for bit."""
+    q = SynthDB(21, 2, 150, 0.668, 30, 400)
+    d = SynthDB(22, 50, 180, 0.668, 30, 700)
+    opts = dict(ssswgt=0.0)
+    orc = oracle(opts)
+    cbp = processing(opts)
+    cbp.set_tuning(1024, rmax, 4)
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    for qp in range(q.nprof):
+        print(f"phase1 ss off: rmax={rmax} query {qp}", flush=True)
+        got = cbp.phase1_keys(qp)
+        exp = expected_keys(orc, q, qp, d)
+        bad = np.nonzero(got != exp)[0]
+        assert bad.size == 0, "query %d: %d/%d pairs differ" % (qp, bad.size, d.nprof)
