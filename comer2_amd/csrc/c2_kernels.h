@@ -126,8 +126,9 @@ struct StatParams {
     int32_t Lq; float qeno; float searchspace; int32_t ssemodel;
     float ung_lambda, ung_K, gap_lambda, gap_K;
 };
+/* crow: the counts row of pair i (nullptr: row i) */
 hipError_t c2_launch_stats(const StatParams& sp, const float* nnw /*device [93]*/,
-                           const int32_t* counts, const StatIn* in, int32_t n,
+                           const int32_t* counts, const int32_t* crow, const StatIn* in, int32_t n,
                            double* out /* [n][10] */, hipStream_t stream);
 hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream_t stream);
 /* persistent workgroups that fill the device once for k_phase1q<R,NW,btck> with lds bytes */

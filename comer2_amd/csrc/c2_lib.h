@@ -87,6 +87,10 @@ struct QWork {
     std::vector<int> passfin;
     pvec<StatIn> si;
     DBuf pass_pro, counts, statin, statout;
+    /* the score histograms of all candidates were launched on a second stream
+     * together with the backtrack pass (a query on its own); counts rows are
+     * then candidate indices and the stats stage reads them through passfin */
+    bool hist_ahead = false;
     pvec<double> hstat;
     std::vector<int> sel;
     pvec<MapPair> mp;

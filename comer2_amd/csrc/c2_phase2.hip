@@ -409,7 +409,7 @@ __device__ double prob_len_jw(double* C, double* NW, const double* pu, const int
  * (CalcStatParamsDynPlmProfile + CalcSignificanceSS18/ObsDynPlmProfile,
  * cuss/CuBatchSS_dynplm.cu:193-674), FP64 */
 __global__ __launch_bounds__(256) void k_stats(StatParams sp, const float* nnw, const int32_t* counts,
-                                               const StatIn* in, int32_t n, double* out)
+                                               const int32_t* crow, const StatIn* in, int32_t n, double* out)
 {
     __shared__ double sP[4][NUNS * SCL], sNW[4][NUNS * SCL], spu[4][NUNS];
     __shared__ int sL[4][NUNS], sR[4][NUNS];
@@ -418,7 +418,7 @@ __global__ __launch_bounds__(256) void k_stats(StatParams sp, const float* nnw, 
     if (i >= n) return;
     double* P = sP[wv]; double* NW = sNW[wv]; double* pu = spu[wv];
     int* L = sL[wv]; int* Rr = sR[wv];
-    const int32_t* cnt = counts + (size_t)i * NSC;
+    const int32_t* cnt = counts + (size_t)(crow ? crow[i] : i) * NSC;
     if (lane < NUNS) {
         int l, r;
         interval(lane, &l, &r);
@@ -593,11 +593,11 @@ hipError_t c2_launch_hist(const ScoreCtx& c, const int32_t* pro, int32_t npro, i
     return hipGetLastError();
 }
 
-hipError_t c2_launch_stats(const StatParams& sp, const float* nnw, const int32_t* counts,
+hipError_t c2_launch_stats(const StatParams& sp, const float* nnw, const int32_t* counts, const int32_t* crow,
                            const StatIn* in, int32_t n, double* out, hipStream_t stream)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stats, dim3((n + 3) / 4), dim3(256), 0, stream, sp, nnw, counts, in, n, out);
+    hipLaunchKernelGGL(k_stats, dim3((n + 3) / 4), dim3(256), 0, stream, sp, nnw, counts, crow, in, n, out);
     return hipGetLastError();
 }
 

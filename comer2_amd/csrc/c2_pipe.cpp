@@ -294,6 +294,20 @@ int cand_prepare(cbp_handle* h, QWork& w, bool launch)
     if ((rc = c2_dalloc(h, w.btck, sizeof(uint32_t) * (size_t)w.off[nc]))) return rc;
     if ((rc = c2_dalloc(h, w.fin, sizeof(FinRec) * nc))) return rc;
     if (!launch) return CBP_OK;
+    if (w.hist_ahead) {
+        /* the histograms depend only on the pairs, not on the backtrace: all
+         * candidates' run on a second stream beside the backtrack pass and
+         * the finalize (latency-bound launches that leave most CUs idle) */
+        if ((rc = ensure_fork_events(h))) return rc;
+        if ((rc = c2_dalloc(h, w.counts, sizeof(int32_t) * 641 * (size_t)nc))) return rc;
+        ScoreCtx hs;
+        c2_fill_scorectx(h, dq, &hs);
+        hipStream_t sA = h->xstream[0];
+        PCHK(hipEventRecord(h->gfork, w.st));
+        PCHK(hipStreamWaitEvent(sA, h->gfork, 0));
+        PCHK(c2_launch_hist(hs, w.dcand.as<int32_t>(), nc, h->opt.xuninf, w.counts.as<int32_t>(), sA));
+        PCHK(hipEventRecord(h->gjoin[0], sA));
+    }
     P1Args a;
     if ((rc = c2_fill_p1args(h, dq, &a))) return rc;
     if ((rc = cand_args(h, w, &a))) return rc;
@@ -395,11 +409,17 @@ int stage_stats(cbp_handle* h, QWork& w)
     if (npass == 0) return CBP_OK;
     if (!h->dnnw.p) return c2_fail(h, CBP_ERR_STATE, "SS18 parameters not uploaded");
     int rc;
-    if ((rc = c2_dupload(h, w.pass_pro, w.pass.data(), npass, w.st))) return rc;
-    if ((rc = c2_dalloc(h, w.counts, sizeof(int32_t) * 641 * (size_t)npass))) return rc;
-    ScoreCtx sc;
-    c2_fill_scorectx(h, dq, &sc);
-    PCHK(c2_launch_hist(sc, w.pass_pro.as<int32_t>(), npass, h->opt.xuninf, w.counts.as<int32_t>(), w.st));
+    if (w.hist_ahead) {
+        /* the passed pairs' rows among the candidates' histograms */
+        if ((rc = c2_dupload(h, w.pass_pro, w.passfin.data(), npass, w.st))) return rc;
+        PCHK(hipStreamWaitEvent(w.st, h->gjoin[0], 0));
+    } else {
+        if ((rc = c2_dupload(h, w.pass_pro, w.pass.data(), npass, w.st))) return rc;
+        if ((rc = c2_dalloc(h, w.counts, sizeof(int32_t) * 641 * (size_t)npass))) return rc;
+        ScoreCtx sc;
+        c2_fill_scorectx(h, dq, &sc);
+        PCHK(c2_launch_hist(sc, w.pass_pro.as<int32_t>(), npass, h->opt.xuninf, w.counts.as<int32_t>(), w.st));
+    }
     w.si.resize(npass);
     for (int k = 0; k < npass; k++) {
         w.si[k].Ldb = ch.len[w.pass[k]];
@@ -416,7 +436,8 @@ int stage_stats(cbp_handle* h, QWork& w)
     sp.ssemodel = h->opt.ssemodel;
     sp.ung_lambda = h->opt.ung_lambda; sp.ung_K = h->opt.ung_K;
     sp.gap_lambda = h->opt.gap_lambda; sp.gap_K = h->opt.gap_K;
-    PCHK(c2_launch_stats(sp, h->dnnw.as<float>(), w.counts.as<int32_t>(), w.statin.as<StatIn>(),
+    PCHK(c2_launch_stats(sp, h->dnnw.as<float>(), w.counts.as<int32_t>(),
+                         w.hist_ahead ? w.pass_pro.as<int32_t>() : nullptr, w.statin.as<StatIn>(),
                          npass, w.statout.as<double>(), w.st));
     PCHK(hipMemcpyAsync(w.hstat.data(), w.statout.p, sizeof(double) * w.hstat.size(), hipMemcpyDeviceToHost, w.st));
     return CBP_OK;
@@ -722,7 +743,10 @@ int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* 
             if ((rc = sync_slots(h, nslots))) return rc;
         }
     }
-    for (int i = 0; i < nq; i++) if ((rc = cand_prepare(h, *h->work[i], !queue))) return rc;
+    for (int i = 0; i < nq; i++) {
+        h->work[i]->hist_ahead = (nq == 1);
+        if ((rc = cand_prepare(h, *h->work[i], !queue))) return rc;
+    }
     if (queue && (rc = queue_cand_run(h, nq))) return rc;
     for (int i = 0; i < nq; i++) if ((rc = cand_finalize(h, *h->work[i]))) return rc;
     if ((rc = sync_slots(h, nslots))) return rc;
