@@ -229,11 +229,12 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
          * advanced by one slot per step */
         const int soff_end = (RING - 1) * ns;
         int soff = ((RING * 4 - lane) % RING) * ns;
-        /* a step's column fields are read at the end of the step before, after
-         * that step's refill: the ring then holds the 64 columns the lanes read
-         * next and the 32 columns of the block after them */
+        /* a step's column fields (with the profile length and id a profile
+         * start needs) are read at the end of the step before, after that
+         * step's refill: the ring then holds the 64 columns the lanes read next
+         * and the 32 columns of the block after them */
         float dw[20], dt[7], pj[3];
-        int xloc;
+        int xloc, nlen, npid;
         uint32_t bb;
         auto read_col = [&]() {
             /* one LDS address for all of the column's fields: opaque, so that
@@ -249,11 +250,12 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
 #pragma unroll
             for (int j = 0; j < 3; j++) pj[j] = rs[F_PJ + j];
             xloc = __float_as_int(rs[F_X]);
+            nlen = __float_as_int(rs[F_LEN]);
+            npid = __float_as_int(rs[F_PID]);
             bb = __float_as_uint(rs[F_BIN]);
         };
         read_col();
         for (int t = 0; t < nsteps; t++) {
-            const float* rs = ring + soff;
 
             /* profile start: flush, reset to the column -1 boundary */
             const bool first = (xloc == 0);
@@ -264,8 +266,8 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
                         const int tb = bpos >> 3, kb = bpos & 7;
                         atomicMax(a.keys + curpid, mkkey(best, (tb - lane) - pstart, y0 + kb));
                     }
-                    const int ldb = __float_as_int(rs[F_LEN]);
-                    curpid = __float_as_int(rs[F_PID]);
+                    const int ldb = nlen;
+                    curpid = npid;
                     pstart = t - lane;
                     best = 0.0f; bpos = 0;
                     const int mx = a.Lq > ldb ? a.Lq : ldb;
