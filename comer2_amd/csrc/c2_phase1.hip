@@ -16,7 +16,11 @@
  *     through one DPP wave_shr:1 per state per step;
  *   - DB column data (20 weighted target frequencies, 7 transitions, 3 SS
  *     probabilities, profile-local x, length, profile id) is staged in a
- *     96-column LDS ring per wave, 32 columns at a time;
+ *     96-column LDS ring per wave, 32 columns at a time, and read by each
+ *     lane one step ahead of its use;
+ *   - the R rows of a step are scored stage by stage across the rows (R
+ *     independent FMA and logarithm chains interleaved in one wave), then
+ *     the Viterbi recurrences run row by row;
  *   - query data (target frequencies, transitions) lives in registers, the
  *     per-row 30-entry SS table QT and the column -1 boundary vectors in LDS
  *     shared by the workgroup;
@@ -256,7 +260,6 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
         };
         read_col();
         for (int t = 0; t < nsteps; t++) {
-
             /* profile start: flush, reset to the column -1 boundary */
             const bool first = (xloc == 0);
             uint32_t fixbits = 0;
