@@ -294,20 +294,6 @@ int cand_prepare(cbp_handle* h, QWork& w, bool launch)
     if ((rc = c2_dalloc(h, w.btck, sizeof(uint32_t) * (size_t)w.off[nc]))) return rc;
     if ((rc = c2_dalloc(h, w.fin, sizeof(FinRec) * nc))) return rc;
     if (!launch) return CBP_OK;
-    if (w.hist_ahead) {
-        /* the histograms depend only on the pairs, not on the backtrace: all
-         * candidates' run on a second stream beside the backtrack pass and
-         * the finalize (latency-bound launches that leave most CUs idle) */
-        if ((rc = ensure_fork_events(h))) return rc;
-        if ((rc = c2_dalloc(h, w.counts, sizeof(int32_t) * 641 * (size_t)nc))) return rc;
-        ScoreCtx hs;
-        c2_fill_scorectx(h, dq, &hs);
-        hipStream_t sA = h->xstream[0];
-        PCHK(hipEventRecord(h->gfork, w.st));
-        PCHK(hipStreamWaitEvent(sA, h->gfork, 0));
-        PCHK(c2_launch_hist(hs, w.dcand.as<int32_t>(), nc, h->opt.xuninf, w.counts.as<int32_t>(), sA));
-        PCHK(hipEventRecord(h->gjoin[0], sA));
-    }
     P1Args a;
     if ((rc = c2_fill_p1args(h, dq, &a))) return rc;
     if ((rc = cand_args(h, w, &a))) return rc;
@@ -320,6 +306,20 @@ int cand_prepare(cbp_handle* h, QWork& w, bool launch)
     int nwb = 2;
     while (nwb > 1 && c2_phase1_lds_bytes(dq->hp.R, nwb, ns) > 80 * 1024) nwb >>= 1;
     PCHK(c2_launch_phase1(a, dq->hp.R, nwb, true, w.st));
+    if (w.hist_ahead) {
+        /* the histograms depend only on the pairs, not on the backtrace: all
+         * candidates' run on a second stream after the backtrack pass, beside
+         * the finalize (one thread per candidate, which leaves most CUs idle) */
+        if ((rc = ensure_fork_events(h))) return rc;
+        if ((rc = c2_dalloc(h, w.counts, sizeof(int32_t) * 641 * (size_t)nc))) return rc;
+        ScoreCtx hs;
+        c2_fill_scorectx(h, dq, &hs);
+        hipStream_t sA = h->xstream[0];
+        PCHK(hipEventRecord(h->gfork, w.st));
+        PCHK(hipStreamWaitEvent(sA, h->gfork, 0));
+        PCHK(c2_launch_hist(hs, w.dcand.as<int32_t>(), nc, h->opt.xuninf, w.counts.as<int32_t>(), sA));
+        PCHK(hipEventRecord(h->gjoin[0], sA));
+    }
     return CBP_OK;
 }
 
