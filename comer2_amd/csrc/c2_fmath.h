@@ -14,7 +14,11 @@
  *   c2_logf : log1p(f) = f + f^2 P7(f), f in [sqrt(1/2)-1, sqrt(2)-1];
  *             <= 1.5 ulp against log() in double over (1e-35, 1e35).
  *   c2_expf : exp(r) = 1 + r + r^2 Q4(r), |r| <= ln2/2, Cody-Waite reduction;
- *             <= 1.02 ulp against exp() in double over [-87, 88].
+ *             <= 1.02 ulp against exp() in double over [-87, 88]; defined
+ *             for x in [-87.68, 88.37]: 2^k is built in the exponent field,
+ *             so k = rint(x log2 e) must lie in [-126, 127] (x >= 88.38
+ *             gives +inf although exp(x) is finite there).  Every caller in
+ *             this library passes x <= 0.
  * The reference uses CUDA fast intrinsics (__logf, __expf) and Schraudolph's
  * approximation for these operations (cusco/CuBatchScoreMatrix_Init.cu:129-177,
  * cucom/myfastapproxmath.cuh:12-40); neither is reproduced here.
@@ -121,7 +125,8 @@ __device__ __forceinline__ void c2_logf_clamped_n(const float* x, float* out) {
 }
 #endif
 
-/* exponential for x in [-87.3, 88.7]; callers clamp or threshold */
+/* exponential for x in [-87.68, 88.37] (see the header); callers clamp or
+ * threshold */
 C2_HD float c2_expf(float x) {
     float kf = rintf(x * 1.4426950408889634f);
     float r = fmaf(kf, -0.693145751953125f, x);

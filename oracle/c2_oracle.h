@@ -111,6 +111,9 @@ double or_lgammainc_P(double a, double x);
 /* pinning hooks: 1 = disable the reference's column-(-1) zero quirk rows
  * (planted mistake used only to show that tests can see it) */
 void or_set_flags(int flags);
+/* c2_logf / c2_expf of c2_fmath.h over an array */
+void or_vlogf(const float* x, float* y, int64_t n);
+void or_vexpf(const float* x, float* y, int64_t n);
 #ifdef __cplusplus
 }
 #endif

@@ -210,6 +210,12 @@ static st5 cell_f32(const st5* dg, const st5* up, const st5* lf,
 static int g_flags = 0;
 void or_set_flags(int flags) { g_flags = flags; }
 
+/* the FP32 elementary functions the kernels and this restatement share
+ * (c2_fmath.h), exposed element-wise so that tests can measure them against
+ * the C library in double precision */
+void or_vlogf(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; i++) y[i] = c2_logf(x[i]); }
+void or_vexpf(const float* x, float* y, int64_t n) { for (int64_t i = 0; i < n; i++) y[i] = c2_expf(x[i]); }
+
 void or_colm1(const cbp_profiles* q, int qp, float (*lleft)[5], float (*ldiag)[5])
 {
     const int noquirk = g_flags & 1;

@@ -387,3 +387,65 @@ def test_map_edge_sizes_match_reference_schedule(orc):
                 assert abs(r1.logZ - r2.logZ) <= 1e-6 * abs(r1.logZ), ctx
             if EDGE_Q[qp] >= 3:
                 assert r1.alnlen == r2.alnlen, ctx
+
+
+
+def _lost_superset(Lq, Ld):
+    """cells whose value the reference's carried phase-1 maximum may never
+    see.  ExecDP_Btck_Unroll32x (cudp/CuBatchDP_init_btck.cu:244-262,
+    435-465) carries each thread's running maximum from column x-1 to column
+    x+ndx of a per-column buffer; FinalizeDPPro reads its last 64 entries
+    (cudp/CuBatchDP_final.cu:117-124).  Two write patterns drop values:
+      * block row 0 (y < 32) writes unconditionally, and where the profile's
+        last blocks shorten ndx (a length not a multiple of 32) a later block
+        overwrites row 0's chain before any block reads it, or carries it to
+        column Ld, which is not written: row 0 is lost (which part depends
+        on Ld mod 32 and the number of 32-row blocks);
+      * for Ld <= 30 the block at x0 = -32 has ndx = Ld and thread t writes
+        column Ld-32+t < 0 for t <= 31-Ld: its cells, rows r with
+        Ld <= r%32 <= 30 at columns c <= 30-r%32, are lost."""
+    s = np.zeros((Lq, Ld), bool)
+    if Ld % 32:
+        s[0, :] = True
+    if Ld <= 30:
+        for r in range(Lq):
+            if Ld <= r % 32 <= 30:
+                s[r, :min(Ld, 31 - r % 32)] = True
+    return s
+
+
+@pytest.mark.parametrize("qlen", [33, 50, 150, 300, 1000])
+def test_reference_maximum_misses_only_the_derived_cells(orc, qlen):
+    """On the literal block-schedule emulation, the reference's maximum is
+    never below the maximum over the cells outside the derived set, for
+    query lengths of every benchmark range (C2's span 50-1000) and DB
+    lengths around the 32-column block edges up to 3,000; the oracle and the
+    kernels take the maximum over all cells.  For Ld >= 30 (every benchmark
+    DB is clipped there) the derived cells are row 0 and, at Ld = 30, column
+    0 of rows with r % 32 == 30: one-cell local alignments, whose value is
+    at most that cell's score (below 5 for real profiles) and never reaches
+    a score threshold (about 24 at C1), so hit lists are unaffected.  Rows
+    of S are raised at random so that the maximum often sits in row 0."""
+    lens = [30, 31, 32, 33, 50, 64, 65, 200, 777, 3000]
+    q = SynthDB(1100 + qlen, 1, qlen, 0.0, qlen, qlen)
+    d = SynthDB.from_lengths(1200 + qlen, lens)
+    differ = 0
+    for dp, Ld in enumerate(lens):
+        for seed in range(3):
+            rng = np.random.default_rng(seed * 7919 + Ld)
+            S = rng.normal(-0.5, 1.0, (qlen, Ld)).astype(np.float32)
+            if seed:
+                S[0, :] += np.float32(3.0 * seed)
+                S[30 % qlen, 0] += np.float32(2.0 * seed)
+            r1, _, mm = orc.viterbi_f32(q.view, 0, d.view, dp, S, want_mm=True)
+            r2, _, _ = orc.refemu(q.view, 0, d.view, dp, S)
+            sup = _lost_superset(qlen, Ld)
+            outside = float(mm[~sup].max(initial=0.0))
+            ctx = (qlen, Ld, seed)
+            assert max(0.0, outside) <= r2.maxsc <= r1.maxsc, ctx
+            assert r1.maxsc == max(0.0, float(mm.max())), ctx
+            # one-cell alignments: no derived cell exceeds its own score
+            assert np.all(mm[sup] <= np.maximum(S[sup], 0.0)), ctx
+            differ += r1.maxsc != r2.maxsc
+    if qlen <= 50:
+        assert differ >= 1      # the deviation is real
