@@ -11,10 +11,12 @@ CSRC = os.path.join(HERE, "csrc")
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
 HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
-HIP_SOURCES = ["c2_phase1.hip", "c2_phase2.hip", "c2_map.hip", "c2_sdp.hip", "c2_lib.cpp", "c2_pipe.cpp", "c2_prep.cpp"]
-HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC", "-shared",
+HIP_SOURCES = ["c2_phase1.hip", "c2_phase2.hip", "c2_map.hip", "c2_sdp.hip", "c2_lib.cpp", "c2_pipe.cpp",
+               "c2_prep.cpp", "c2_fields.cpp", "c2_batchproc.cpp"]
+HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]
+OBJDIR = os.path.join(HERE, "_build")
 
 
 def _run(cmd):
@@ -33,10 +35,16 @@ def build_synth():
 
 
 def build_hip():
+    """one object per source, compiled in parallel, then linked"""
+    from concurrent.futures import ThreadPoolExecutor
     out = os.path.join(HERE, "libcomer2_hip.so")
     tmp = out + ".tmp"
-    srcs = [os.path.join(CSRC, s) for s in HIP_SOURCES]
-    _run([HIPCC] + HIP_FLAGS + ["-o", tmp] + srcs)
+    os.makedirs(OBJDIR, exist_ok=True)
+    objs = [os.path.join(OBJDIR, os.path.splitext(s)[0] + ".o") for s in HIP_SOURCES]
+    jobs = [[HIPCC] + HIP_FLAGS + ["-c", "-o", o, os.path.join(CSRC, s)] for s, o in zip(HIP_SOURCES, objs)]
+    with ThreadPoolExecutor(min(8, len(jobs))) as ex:
+        list(ex.map(_run, jobs))
+    _run([HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp] + objs)
     os.replace(tmp, out)
     return out
 

@@ -42,12 +42,12 @@ void HostSet::copy_from(const cbp_profiles* s) {
 
 int c2_dalloc(cbp_handle* h, DBuf& b, size_t n) {
     if (b.p && b.n >= n) return CBP_OK;
-    if (b.p) { h->used -= b.n; (void)hipFree(b.p); b.p = nullptr; b.n = 0; }
+    b.release();
     if (n == 0) n = 8;
     if (h->budget && h->used + n > h->budget)
         return fail(h, CBP_ERR_MEM, "device memory budget exceeded");
     HIPCHK(hipMalloc(&b.p, n));
-    b.n = n; h->used += n;
+    b.n = n; b.acct = &h->used; h->used += n;
     return CBP_OK;
 }
 
@@ -168,6 +168,54 @@ int cbp_upload_queries(cbp_handle* h, const cbp_profiles* q)
 
 }  /* extern "C" */
 
+/* appends one profile to the query set; the preparations of the queries
+ * already there stay cached (their indices and data are unchanged) */
+int c2_append_query(cbp_handle* h, const cbp_profiles* one, int32_t* qidx)
+{
+    if (!check_set(one) || one->nprof != 1) return fail(h, CBP_ERR_ARG, "invalid query profile");
+    if (!h->have_queries) {
+        const int rc = cbp_upload_queries(h, one);
+        if (rc == CBP_OK) *qidx = 0;
+        return rc;
+    }
+    const cbp_profiles& o = h->queries.v;
+    const size_t np = (size_t)o.nprof, ns = (size_t)o.npos, L = (size_t)one->npos;
+    const size_t np2 = np + 1, ns2 = ns + L;
+    std::vector<float> bkg(20 * np2), eno(np2), trg(20 * ns2), ss(3 * ns2), tr(7 * (ns2 + np2));
+    std::vector<int32_t> len(np2);
+    std::vector<uint32_t> dist(np2);
+    std::vector<char> aa(ns2), sss(ns2);
+    for (size_t a = 0; a < 20; a++) {
+        std::copy(o.bkg + a * np, o.bkg + (a + 1) * np, bkg.begin() + a * np2);
+        bkg[a * np2 + np] = one->bkg[a];
+        std::copy(o.trgfrq + a * ns, o.trgfrq + (a + 1) * ns, trg.begin() + a * ns2);
+        std::copy(one->trgfrq + a * L, one->trgfrq + (a + 1) * L, trg.begin() + a * ns2 + ns);
+    }
+    for (size_t k = 0; k < 3; k++) {
+        std::copy(o.ssprb + k * ns, o.ssprb + (k + 1) * ns, ss.begin() + k * ns2);
+        std::copy(one->ssprb + k * L, one->ssprb + (k + 1) * L, ss.begin() + k * ns2 + ns);
+    }
+    for (size_t k = 0; k < 7; k++) {
+        std::copy(o.trans + k * (ns + np), o.trans + (k + 1) * (ns + np), tr.begin() + k * (ns2 + np2));
+        std::copy(one->trans + k * (L + 1), one->trans + (k + 1) * (L + 1), tr.begin() + k * (ns2 + np2) + ns + np);
+    }
+    std::copy(o.eno, o.eno + np, eno.begin()); eno[np] = one->eno[0];
+    std::copy(o.len, o.len + np, len.begin()); len[np] = one->len[0];
+    std::copy(o.dist, o.dist + np, dist.begin()); dist[np] = (uint32_t)ns;
+    std::copy(o.aa, o.aa + ns, aa.begin()); std::copy(one->aa, one->aa + L, aa.begin() + ns);
+    std::copy(o.sss, o.sss + ns, sss.begin()); std::copy(one->sss, one->sss + L, sss.begin() + ns);
+    cbp_profiles v;
+    v.nprof = (int32_t)np2; v.npos = (int64_t)ns2;
+    v.bkg = bkg.data(); v.eno = eno.data(); v.len = len.data(); v.dist = dist.data();
+    v.trgfrq = trg.data(); v.ssprb = ss.data(); v.aa = aa.data(); v.sss = sss.data(); v.trans = tr.data();
+    h->queries.copy_from(&v);
+    h->chunk_gen++;
+    h->sdq_ready = false;
+    h->sdp_ready = false;
+    *qidx = (int32_t)np;
+    return CBP_OK;
+}
+
 /* segments of whole profiles of about seg_cols columns each (the unit of
  * work of one phase-1 wave) */
 int c2_build_segments(cbp_handle* h, const cbp_profiles* s)
@@ -261,7 +309,20 @@ int cbp_score_threshold(const cbp_options* o, uint64_t qlen, uint64_t dblen,
 DevQuery* c2_get_query(cbp_handle* h, int qidx, int* rc)
 {
     auto it = h->qcache.find(qidx);
-    if (it != h->qcache.end()) return it->second.get();
+    if (it != h->qcache.end()) { it->second.tick = h->qtick; return it->second.dq.get(); }
+    /* evict the least recently used preparation of an earlier call; graphs
+     * hold the evicted buffers' addresses, so they retire with a new
+     * generation */
+    while (h->qcache.size() >= C2_QCACHE_MAX) {
+        auto lru = h->qcache.end();
+        for (auto e = h->qcache.begin(); e != h->qcache.end(); ++e)
+            if (e->second.tick < h->qtick && (lru == h->qcache.end() || e->second.tick < lru->second.tick)) lru = e;
+        if (lru == h->qcache.end()) break;      /* all belong to the running call */
+        (void)hipStreamSynchronize(h->stream);
+        h->qcache.erase(lru);
+        c2_drop_graphs(h);
+        h->chunk_gen++;
+    }
     auto dq = std::make_unique<DevQuery>();
     c2_prepare_query(&h->queries.v, qidx, h->ssss.data(), h->rmax, &dq->hp);
     const QueryPrep& p = dq->hp;
@@ -282,7 +343,9 @@ DevQuery* c2_get_query(cbp_handle* h, int qidx, int* rc)
         if ((*rc = dupload(h, dq->qaa, qa.data(), qa.size()))) return nullptr;
     }
     DevQuery* r = dq.get();
-    h->qcache[qidx] = std::move(dq);
+    cbp_handle::QEnt& e = h->qcache[qidx];
+    e.dq = std::move(dq);
+    e.tick = h->qtick;
     *rc = CBP_OK;
     return r;
 }
@@ -314,6 +377,7 @@ int c2_run_phase1(cbp_handle* h, int qidx)
     if (qidx < 0 || qidx >= h->queries.v.nprof) return fail(h, CBP_ERR_ARG, "query index out of range");
     (void)hipSetDevice(h->device);
     int rc;
+    h->qtick++;
     DevQuery* dq = c2_get_query(h, qidx, &rc);
     if (!dq) return rc;
     P1Args a;
@@ -445,20 +509,20 @@ extern "C" int cbp_score_block(cbp_handle* h, int32_t qidx, int32_t pronr, float
     if (pronr < 0 || pronr >= h->dchunk.nprof) return fail(h, CBP_ERR_ARG, "profile index out of range");
     (void)hipSetDevice(h->device);
     int rc;
+    h->qtick++;
     DevQuery* dq = c2_get_query(h, qidx, &rc);
     if (!dq) return rc;
     const int Ld = h->chunk.v.len[pronr];
     const size_t n = (size_t)dq->hp.Lq * Ld;
     DBuf dS, dM;
     if ((rc = dalloc(h, dS, sizeof(float) * n))) return rc;
-    if (mod && (rc = dalloc(h, dM, sizeof(float) * n))) { h->used -= dS.n; return rc; }
+    if (mod && (rc = dalloc(h, dM, sizeof(float) * n))) return rc;
     ScoreCtx sc;
     c2_fill_scorectx(h, dq, &sc);
     hipError_t e = c2_launch_score_block(sc, pronr, Ld, dS.as<float>(), mod ? dM.as<float>() : nullptr, h->stream);
     if (e == hipSuccess) e = hipMemcpyAsync(S, dS.p, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess && mod) e = hipMemcpyAsync(mod, dM.p, sizeof(float) * n, hipMemcpyDeviceToHost, h->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(h->stream);
-    h->used -= dS.n + dM.n;
     if (e != hipSuccess) return fail(h, CBP_ERR_HIP, std::string("cbp_score_block: ") + hipGetErrorString(e));
     return CBP_OK;
 }

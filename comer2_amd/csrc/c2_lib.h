@@ -13,10 +13,26 @@
 #include "c2_prep.h"
 #include "c2_kernels.h"
 
+/* device buffer; c2_dalloc charges its size to the handle's account
+ * (acct), and the release returns it, so the memory budget sees only live
+ * buffers */
 struct DBuf {
     void* p = nullptr;
     size_t n = 0;
-    ~DBuf() { if (p) (void)hipFree(p); }
+    size_t* acct = nullptr;
+    DBuf() = default;
+    DBuf(const DBuf&) = delete;
+    DBuf& operator=(const DBuf&) = delete;
+    DBuf(DBuf&& o) noexcept : p(o.p), n(o.n), acct(o.acct) { o.p = nullptr; o.n = 0; o.acct = nullptr; }
+    DBuf& operator=(DBuf&& o) noexcept {
+        if (this != &o) { release(); p = o.p; n = o.n; acct = o.acct; o.p = nullptr; o.n = 0; o.acct = nullptr; }
+        return *this;
+    }
+    void release() {
+        if (p) { (void)hipFree(p); if (acct) *acct -= n; }
+        p = nullptr; n = 0; acct = nullptr;
+    }
+    ~DBuf() { release(); }
     template <class T> T* as() const { return (T*)p; }
 };
 
@@ -118,6 +134,10 @@ struct QWork {
 };
 
 constexpr int C2_NSTREAMS = 4;
+constexpr size_t C2_QCACHE_MAX = 128;
+/* one-query calls run the candidates' histograms ahead of the finalize when
+ * there are at most this many candidates (641 counts of 4 B each) */
+constexpr int C2_HIST_AHEAD_MAX = 16384;
 
 struct cbp_handle {
     int device = 0;
@@ -132,7 +152,11 @@ struct cbp_handle {
     bool have_tables = false;
     HostSet queries;
     bool have_queries = false;
-    std::map<int, std::unique_ptr<DevQuery>> qcache;
+    /* device query preparations, least recently used evicted beyond
+     * C2_QCACHE_MAX entries (never one of the running call's queries) */
+    struct QEnt { std::unique_ptr<DevQuery> dq; uint64_t tick = 0; };
+    std::map<int, QEnt> qcache;
+    uint64_t qtick = 0;          /* bumped once per call that prepares queries */
     HostSet chunk;
     DevSet dchunk;
     bool have_chunk = false;
@@ -207,6 +231,8 @@ inline int c2_dupload(cbp_handle* h, DBuf& b, const T* src, size_t count, hipStr
     return CBP_OK;
 }
 DevQuery* c2_get_query(cbp_handle* h, int qidx, int* rc);
+int c2_append_query(cbp_handle* h, const cbp_profiles* one, int32_t* qidx);
+void c2_drop_graphs(cbp_handle* h);
 int c2_fill_p1args(cbp_handle* h, DevQuery* dq, P1Args* a);
 void c2_fill_scorectx(cbp_handle* h, DevQuery* dq, ScoreCtx* c);
 uint64_t c2_length_adjustment(const cbp_options& o, uint64_t m, uint64_t n, uint64_t N, uint64_t* delta);

@@ -279,6 +279,9 @@ int cand_prepare(cbp_handle* h, QWork& w, bool launch)
         if (w.hk[p] && u2f((uint32_t)(w.hk[p] >> 32)) > w.scorethld) w.cand.push_back(p);
     const int nc = (int)w.cand.size();
     w.hfin.resize(nc);
+    /* histograms ahead of the finalize cover every candidate: only while the
+     * candidates are few (a low score threshold can pass most of the DB) */
+    if (nc > C2_HIST_AHEAD_MAX) w.hist_ahead = false;
     if (nc == 0) return CBP_OK;
     const int ns = dq->hp.nstripes;
     w.off.assign(nc + 1, 0);
@@ -406,7 +409,12 @@ int stage_stats(cbp_handle* h, QWork& w)
     uint64_t delta = 0;
     w.sspace = c2_length_adjustment(h->opt, (uint64_t)dq->hp.Lq, dblen, nseqs, &delta);
     w.hstat.resize((size_t)10 * npass);
-    if (npass == 0) return CBP_OK;
+    if (npass == 0) {
+        /* the candidates' histograms may still run on the second stream:
+         * the query's stream waits for them before the call returns */
+        if (w.hist_ahead && nc > 0) PCHK(hipStreamWaitEvent(w.st, h->gjoin[0], 0));
+        return CBP_OK;
+    }
     if (!h->dnnw.p) return c2_fail(h, CBP_ERR_STATE, "SS18 parameters not uploaded");
     int rc;
     if (w.hist_ahead) {
@@ -660,13 +668,15 @@ int build_graph(cbp_handle* h, int nq, int nslots, bool queue, hipGraphExec_t* o
     return CBP_OK;
 }
 
-void drop_graphs(cbp_handle* h)
+void drop_graphs(cbp_handle* h) { c2_drop_graphs(h); }
+
+}  // namespace
+
+void c2_drop_graphs(cbp_handle* h)
 {
     for (auto& g : h->graphs) if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
     h->graphs.clear();
 }
-
-}  // namespace
 
 int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* scorethld,
                    const float* logevthld, cbp_results* outs)
@@ -682,6 +692,7 @@ int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* 
     const int nslots = nq < C2_NSTREAMS ? nq : C2_NSTREAMS;
     int rc;
     /* query preparation (host tables uploaded on the handle's stream) */
+    h->qtick++;
     for (int i = 0; i < nq; i++) {
         QWork& w = *h->work[i];
         w.qidx = qidx[i];

@@ -118,6 +118,43 @@ int  cbp_upload_tables(cbp_handle* h, const float* ssss900, const float* nnw, in
  * CopyCPMDataToDevice, cupro/CuDeviceMemory.cu:251, 1036-1104) */
 int  cbp_upload_queries(cbp_handle* h, const cbp_profiles* q);
 int  cbp_upload_chunk(cbp_handle* h, const cbp_profiles* db);
+/* the reference's per-field buffers (char* arrays of pmv2DTotFlds begin and
+ * end pointers, cupro/PM2DVectorFields.h:41-81), as CuBatchProcessing::
+ * TransferCPMData(bdbCpmbeg, bdbCpmend, szCpm2dvfields) receives them for a
+ * DB chunk (cupro/CuBatchProcessing.cuh:60-68): field f spans
+ * [beg[f], end[f]) -- per profile: 20 background probabilities, ENO (float),
+ * length (int), distance (unsigned, positions before the profile); per
+ * profile position plus one beginning row: 7 log transitions; per position:
+ * 20 target frequencies, 19 CV entries, CV prior, CV norm, 3 SS
+ * probabilities, HDP probability, HDP index, 3 addresses (int), residue
+ * (char), SS state (char).  CV, HDP and address fields may be empty; the
+ * distances may start at any offset (they are re-based to the first
+ * profile). */
+#define CBP_PM_NFIELDS 81
+/* profile and position counts of a field set (host only, no GPU) */
+int  cbp_fields_count(const void* const* beg, const void* const* end, int32_t* nprof, int64_t* npos);
+/* gather a field set into the field-major arrays of cbp_profiles, sized by
+ * cbp_fields_count (host only, no GPU) */
+int  cbp_fields_pack(const void* const* beg, const void* const* end,
+                     float* bkg, float* eno, int32_t* len, uint32_t* dist,
+                     float* trgfrq, float* ssprb, char* aa, char* sss, float* trans);
+/* cbp_upload_chunk of a field set (CuBatchProcessing::TransferCPMData,
+ * cupro/CuBatchProcessing.cuh:60-68 -> CuDeviceMemory::CopyCPMDataToDevice,
+ * cupro/CuDeviceMemory.cu:1036-1104) */
+int  cbp_upload_chunk_fields(cbp_handle* h, const void* const* beg, const void* const* end);
+/* a DB of two parts searched as one profile set: the cached part bdb1
+ * (beg1, end1; may be NULL) followed by the chunk bdbC (beg2, end2), the
+ * two parts ProcessScoreMatrix receives (cupro/CuBatchProcessing.cuh:82-94,
+ * numbered as cudp/CuBatchDP_init_btck.cu:96-110 numbers them) */
+int  cbp_upload_db_fields(cbp_handle* h, const void* const* beg1, const void* const* end1,
+                          const void* const* beg2, const void* const* end2);
+/* appends one query, given as the reference's querypmbeg field pointers
+ * advanced to it (cualn/DevCommThread.cpp:312-345) and its length nqyposs,
+ * to the query set; *qidx receives its index; the other queries keep their
+ * indices and device preparations (the query argument of
+ * CuBatchProcessing::ProcessScoreMatrix, cupro/CuBatchProcessing.cuh:82-94) */
+int  cbp_add_query_fields(cbp_handle* h, const void* const* beg, uint64_t nqyposs, int32_t* qidx);
+
 /* whole-database size used for thresholds and E-values
  * (CuBatchProcessing::SetDbDetails) */
 int  cbp_set_db_details(cbp_handle* h, uint64_t dblen, uint64_t nseqs);
