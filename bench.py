@@ -85,25 +85,54 @@ def read_traffic():
     return float(m.group(1)) if m else None
 
 
-def cpu_baseline(q, d, orc, thld, logev, sspace, seconds, nthreads):
-    import numpy as np
-    # strided sample keeps the length distribution of the sorted DB
-    order = np.arange(d.nprof, dtype=np.int32)
-    stride = 7
-    sample = np.concatenate([order[k::stride] for k in range(stride)])
-    cells = mcells = npass = nmap = 0
-    done = 0
+def _cpu_sample(q, d, orc, pros, thld, logev, sspace, seconds, nthreads):
+    """the oracle's whole-pipeline restatement over the listed DB profiles
+    (OpenMP over profiles) until done or out of time"""
+    cells = mcells = npass = nmap = done = 0
     t0 = time.perf_counter()
-    while done < sample.size and time.perf_counter() - t0 < seconds:
-        batch = sample[done:done + 1024]
+    while done < pros.size and time.perf_counter() - t0 < seconds:
+        batch = pros[done:done + 1024]
         r = orc.search(q.view, 0, d.view, batch, thld, logev, float(sspace), nthreads)
         cells += r.cells; mcells += r.map_cells; npass += r.npassed; nmap += r.nmap
         done += batch.size
     dt = time.perf_counter() - t0
-    return dict(value=cells / dt / 1e9, unit="GCUPS", cores=nthreads, kind="port",
-                sample="%d of %d DB profiles (stride-%d interleave), %d cells, %d passed, "
-                       "%d realigned, %.1f s; oracle FP32 restatement, OpenMP"
-                       % (done, d.nprof, stride, cells, npass, nmap, dt))
+    return dict(gcups=cells / dt / 1e9, done=done, cells=cells, passed=npass, realigned=nmap, seconds=dt)
+
+
+def cpu_baseline(q, d, orc, thld, logev, sspace, seconds, nthreads):
+    """C1 on the host: the oracle's FP32 restatement of the whole pipeline on
+    a strided sample of the DB, on nthreads and on one thread; and C0 (one
+    query of 150 positions vs 2,000 profiles, seed 1, MAP of the best 100)
+    whole, on nthreads and on one thread"""
+    import numpy as np
+    from comer2_amd._ffi import SynthDB
+    order = np.arange(d.nprof, dtype=np.int32)
+    stride = 7
+    sample = np.concatenate([order[k::stride] for k in range(stride)])
+    allc = _cpu_sample(q, d, orc, sample, thld, logev, sspace, seconds, nthreads)
+    one = _cpu_sample(q, d, orc, sample, thld, logev, sspace, max(2.0, seconds / 4), 1)
+    # C0: seed 1, query 150, 2,000 profiles (median 180, clip 30-800), MAP top 100
+    q0 = SynthDB(1001, 1, 150, 0.0, 150, 150)
+    d0 = SynthDB(1, 2000, 180, SIGMA, 30, 800)
+    d0.plant(q0, 3001, 40, PLANT_KMIN, PLANT_KMAX)
+    t0, l0 = orc.score_threshold(150, d0.npos)
+    ss0, _ = orc.length_adjustment(150, d0.npos, d0.nprof)
+    every = np.arange(d0.nprof, dtype=np.int32)
+    c0n = _cpu_sample(q0, d0, orc, every, t0, l0, ss0, 60.0, nthreads)
+    c01 = _cpu_sample(q0, d0, orc, every, t0, l0, ss0, 60.0, 1)
+    return dict(value=allc["gcups"], unit="GCUPS", cores=nthreads, kind="port",
+                sample="C1: %d of %d DB profiles (stride-%d interleave), %d cells, %d passed, %d realigned, "
+                       "%.1f s; oracle FP32 restatement of the whole pipeline, OpenMP over profiles"
+                       % (allc["done"], d.nprof, stride, allc["cells"], allc["passed"], allc["realigned"],
+                          allc["seconds"]),
+                nproc=os.cpu_count(), affinity=len(os.sched_getaffinity(0)),
+                omp_num_threads=os.environ.get("OMP_NUM_THREADS"),
+                one_core={"value": one["gcups"], "sample": "C1: %d profiles, %d cells, %.1f s"
+                          % (one["done"], one["cells"], one["seconds"])},
+                c0={"profiles": d0.nprof, "positions": int(d0.npos), "query_len": 150, "seed": 1,
+                    "planted": 40, "passed": c0n["passed"], "realigned": c0n["realigned"],
+                    "gcups": c0n["gcups"], "seconds": round(c0n["seconds"], 3),
+                    "one_core_gcups": c01["gcups"], "one_core_seconds": round(c01["seconds"], 3)})
 
 
 def batch_bench(d, args):
@@ -128,7 +157,10 @@ def batch_bench(d, args):
     ttune = time.perf_counter() - t
     log("bench: autotuned on query %d (L=%d): %d columns per segment, %d waves per workgroup, "
         "at most %d rows per lane (%.1f s)" % (mid, int(lens[mid]), seg_cols, nw, rmax, ttune))
-    cbp.ProcessBatch(qs[:4])                          # warm-up: buffers, query tables
+    # warm-up of every leg's state before its timing: the one-by-one leg
+    # prepares all 64 queries and grows the one-query buffers here
+    for i in qs:
+        cbp.ProcessScoreMatrix(i)
     t = time.perf_counter()
     plain = [cbp.ProcessScoreMatrix(i) for i in qs]
     tp = time.perf_counter() - t
@@ -349,10 +381,14 @@ def main():
         from comer2_amd import tables
         orc = Oracle(tables.load_ssss(), tables.load_nn_weights(),
                      dict(tables.default_options(), max_map_pairs=MAP_CAP))
-        nthreads = max(1, min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0))))
+        # every core this process may use, or the share OMP_NUM_THREADS grants
+        nthreads = max(1, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or len(os.sched_getaffinity(0)))
         out["cpu_baseline"] = cpu_baseline(q, d, orc, thld, logev, res["search_space"],
                                            args.cpu_seconds, nthreads)
-        log("bench: cpu baseline %.4f GCUPS on %d threads" % (out["cpu_baseline"]["value"], nthreads))
+        cb = out["cpu_baseline"]
+        log("bench: cpu baseline C1 %.4f GCUPS on %d threads, %.4f on one; C0 %.4f / %.4f GCUPS (%d passed)"
+            % (cb["value"], nthreads, cb["one_core"]["value"], cb["c0"]["gcups"], cb["c0"]["one_core_gcups"],
+               cb["c0"]["passed"]))
     print(json.dumps(out), flush=True)
 
 
