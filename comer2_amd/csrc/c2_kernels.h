@@ -82,7 +82,8 @@ struct MapArgs {
     const float* qtrb;               /* query backward transitions (all at row+1) [7][Lpad] */
     const float* mleft; const float* mdiag;  /* [5][Lq] */
     float* Sd;                       /* dynamic scores S' (lane-step layout) */
-    float* F;                        /* F, then F+B (lane-step layout) */
+    float* F;                        /* forward F_MM (lane-step layout) */
+    float* B;                        /* backward B_MM (lane-step layout); the MAP DP adds F + B */
     float* hand;                     /* stripe hand-offs */
     char* btck;
     float* stat;                     /* [npairs][4]: maxF, logZ, -, flag */
@@ -102,9 +103,27 @@ struct MapArgs {
 };
 /* columns per item of the dynamic-score pass */
 constexpr int C2_MAP_SCORE_COLS = 256;
+/* MAPALN=0 drawing (FinalizeDP_ALN): the phase-1 backtrace of each drawn
+ * pair from the phase-1 maximum through the candidates' backtrack codes */
+struct AlnArgs {
+    const int32_t* pro;              /* [n] chunk profile of each drawn pair */
+    const int32_t* cand;             /* [n] its candidate index (code offset) */
+    const unsigned long long* keys;  /* [nprof] phase-1 maxima */
+    const uint32_t* btck;            /* candidates' packed backtrack codes */
+    const int64_t* btck_off;         /* [ncand] word offset of each candidate's codes */
+    char* alns; char* alntmp;        /* [n][3][alnstride] */
+    int64_t alnstride;
+    int32_t* res;                    /* [n][8] as MapArgs::res (slots 0..5) */
+};
+hipError_t c2_launch_aln_final(const ScoreCtx& c, const AlnArgs& a, int n, hipStream_t stream);
 /* MAP geometry for a query: rows per lane and waves per pair */
 void c2_map_geometry(int Lq, int* R, int* ns, int* nw);
-hipError_t c2_launch_map(const ScoreCtx& c, const MapArgs& m, int npairs, hipStream_t stream);
+/* score pass, then the forward pass and log Z on stream while the backward
+ * pass runs on bstream (the two read only S'), then the MAP DP and the
+ * backtrace on stream; bstream == nullptr runs every pass on stream; fork and
+ * join are events of the handle (unused without bstream) */
+hipError_t c2_launch_map(const ScoreCtx& c, const MapArgs& m, int npairs, hipStream_t stream,
+                         hipStream_t bstream, hipEvent_t fork, hipEvent_t join);
 /* floats of a pair's lane-step matrix (stripes x (Ld+63) steps x 64R) and of its hand-off buffers */
 int64_t c2_map_cells(int R, int nstripes, int Ld);
 int64_t c2_map_hand(int nstripes, int Ld);

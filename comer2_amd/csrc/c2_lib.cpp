@@ -105,12 +105,22 @@ int cbp_destroy(cbp_handle* h)
     h->graphs.clear();
     if (h->gfork) (void)hipEventDestroy(h->gfork);
     for (int s = 0; s < C2_NSTREAMS - 1; s++) if (h->gjoin[s]) (void)hipEventDestroy(h->gjoin[s]);
+    for (int s = 0; s < 2; s++) {
+        if (h->mfork[s]) (void)hipEventDestroy(h->mfork[s]);
+        if (h->mjoin[s]) (void)hipEventDestroy(h->mjoin[s]);
+    }
     if (h->gt0) (void)hipEventDestroy(h->gt0);
     if (h->gt1) (void)hipEventDestroy(h->gt1);
     if (h->hqkeys) (void)hipHostFree(h->hqkeys);
     h->work.clear();
     h->qcache.clear();
     h->dchunk = DevSet();
+    for (int s = 0; s < 2; s++) {
+        h->dsets[s] = DevSet();
+        if (h->stage[s]) (void)hipHostFree(h->stage[s]);
+        if (h->up_ev[s]) (void)hipEventDestroy(h->up_ev[s]);
+    }
+    if (h->ustream) (void)hipStreamDestroy(h->ustream);
     for (int s = 0; s < C2_NSTREAMS - 1; s++)
         if (h->xstream[s]) (void)hipStreamDestroy(h->xstream[s]);
     if (h->ev0) (void)hipEventDestroy(h->ev0);
@@ -764,11 +774,12 @@ extern "C" int cbp_chunk_limits(cbp_handle* h, uint64_t qlen, uint64_t* maxpos, 
     const double per_pos = 4.0 * (20 + 3 + 7) + 1 + 1 + 4 + (ns > 1 ? 20.0 : 0.0);
     const double per_pro = 4.0 * (20 + 1 + 1 + 1 + 7 + 2 + 20);
     const double p1 = per_pos + per_pro / expct_len;
-    /* pass 2, per DB position of a realigned pair: S', F (4 B each) and MAP
-     * backtrack (1 B) over 64*R*stripes rows; the candidates' 2-bit phase-1
-     * backtrack codes (4 B per step and lane per stripe); hand-off rows */
+    /* pass 2, per DB position of a realigned pair: S', F, B (4 B each) and
+     * MAP backtrack (1 B) over 64*R*stripes rows; the candidates' 2-bit
+     * phase-1 backtrack codes (4 B per step and lane per stripe); hand-off
+     * rows */
     const double rows_m = 64.0 * mR * mns;
-    const double p2 = 9.0 * rows_m + 4.0 * 64.0 * ns + 4.0 * (5 * mns + 4);
+    const double p2 = 13.0 * rows_m + 4.0 * 64.0 * ns + 4.0 * (5 * mns + 4);
     double n = (double)avail / (p1 + pass2 * p2);
     uint64_t mp = (uint64_t)n;
     mp &= ~(uint64_t)31;                          /* positions padded to 32, as the reference */

@@ -113,7 +113,9 @@ struct QWork {
     pvec<int32_t> morder;
     pvec<int4> msitems;                 /* dynamic-score work items */
     int64_t alnstride = 0;
-    DBuf mpairs, dmorder, dmsitems, mF, mSd, mhand, mbt, mstat, malns, malntmp, mres;
+    DBuf mpairs, dmorder, dmsitems, mF, mB, mSd, mhand, mbt, mstat, malns, malntmp, mres;
+    pvec<int32_t> apro, acand;          /* MAPALN=0: drawn pairs' profiles and candidate indices */
+    DBuf dapro, dacand;
     pvec<int32_t> hmres;
     int64_t nposits = 0;
     int32_t maxlen = 0;
@@ -125,12 +127,20 @@ struct QWork {
     char* strs = nullptr;               /* pinned host copy of the alignment strings */
     size_t strs_cap = 0;
     std::vector<int> map;
+    std::vector<float> p1save;          /* [nmap][3] phase-1 PSTVS, BEG, END of the drawn rows */
     ~QWork() {
         if (e0) (void)hipEventDestroy(e0);
         if (e1) (void)hipEventDestroy(e1);
         if (hkeys) (void)hipHostFree(hkeys);
         if (strs) (void)hipHostFree(strs);
     }
+};
+
+/* merged result of one query of a whole-DB search in chunks */
+struct StreamOut {
+    std::vector<float> aln, epa;
+    std::vector<int32_t> order, map_rows;
+    std::vector<char> strs;
 };
 
 constexpr int C2_NSTREAMS = 4;
@@ -205,7 +215,19 @@ struct cbp_handle {
     int32_t seg_maxcols = 0;            /* columns of the widest segment */
     hipEvent_t gfork = nullptr, gjoin[C2_NSTREAMS - 1] = {nullptr, nullptr, nullptr};
     hipEvent_t gt0 = nullptr, gt1 = nullptr;
+    /* fork/join of the MAP backward pass beside the forward pass (one-query
+     * calls: the long-pair group and the rest) */
+    hipEvent_t mfork[2] = {nullptr, nullptr}, mjoin[2] = {nullptr, nullptr};
     uint64_t chunk_gen = 0;             /* bumped by every chunk upload / tuning change */
+    /* whole-DB search in chunks: two device chunk sets, pinned staging
+     * buffers and the copy stream of the copy thread, merged results */
+    DevSet dsets[2];
+    char* stage[2] = {nullptr, nullptr};
+    size_t stage_n[2] = {0, 0};
+    hipEvent_t up_ev[2] = {nullptr, nullptr};
+    hipStream_t ustream = nullptr;
+    std::vector<StreamOut> smerged;
+    int32_t last_nchunks = 0;
     /* soft DP: query set on the device and the prepared pair plan */
     DBuf sdq_trans, sdq_dist, sdq_ll, sdq_ld, sdq_trg, sdq_ss, sdq_bkg, dssss;
     bool sdq_ready = false;

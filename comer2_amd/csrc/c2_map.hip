@@ -16,9 +16,12 @@
  * lockstep 64 steps apart and pass their boundary row through LDS; bands
  * pass it through per-pair HBM buffers.  R is chosen from the query length
  * so that one band usually covers the query: the critical path of a pair is
- * then about Ld + 64*(stripes-1) steps of R cells.  Per-cell matrices (dynamic score S', forward F,
- * F+B, backtrack codes) live in a "lane-step" layout [stripe][step][row in
- * lane][lane], so every pass reads and writes them coalesced.
+ * then about Ld + 64*(stripes-1) steps of R cells.  Per-cell matrices
+ * (dynamic score S', forward F, backward B, backtrack codes) live in a
+ * "lane-step" layout [stripe][step][row in lane][lane], so every pass reads
+ * and writes them coalesced.  The backward pass depends only on S', so it
+ * runs beside the forward pass and log Z; the MAP DP adds F + B (the
+ * reference adds B into F in place, the same FP32 sum).
  *
  * Arithmetic: every cell evaluates the reference kernels' recurrences with
  * their boundary values and thresholds in the same FP32 operation order as
@@ -27,7 +30,7 @@
  * cucom/myfastapproxmath.cuh), so results match it bit for bit whatever the
  * schedule; log Z is summed in the reference's block order.
  */
-#include "c2_score.h"
+#include "c2_draw.h"
 
 namespace {
 
@@ -461,10 +464,12 @@ __global__ __launch_bounds__(256) void k_map_total(ScoreCtx c, MapArgs m)
     if (threadIdx.x == 0) m.stat[pi * 4 + 1] = mlog(total) + maxF;   /* logZ */
 }
 
-/* ---- backward: F := F + B_MM (in place) ----
+/* ---- backward: B_MM of every cell ----
  * Bands run bottom-up; inside a band wave w runs 64 steps behind wave w+1
  * (reverse sweep), which hands its first row (B_MM, B_MI, B_DN) up through a
- * double-buffered LDS slot; bands hand over through the HBM buffer. */
+ * double-buffered LDS slot; bands hand over through the HBM buffer.  The
+ * pass reads only S' and the transitions, so it runs beside the forward
+ * pass; the MAP DP adds F + B. */
 template <int R>
 __global__ __launch_bounds__(512) void k_map_bwd(ScoreCtx c, MapArgs m)
 {
@@ -476,7 +481,7 @@ __global__ __launch_bounds__(512) void k_map_bwd(ScoreCtx c, MapArgs m)
     const size_t ntr = (size_t)(c.npos + c.nprof);
     const size_t db = (size_t)c.dist[p] + p;
     const float* Sd = m.Sd + mp.foff;
-    float* F = m.F + mp.foff;
+    float* Bm = m.B + mp.foff;
     float* bh = m.hand + mp.hoff + (size_t)ns * 5 * Ld;   /* [3][Ld]: B_MM, B_MI, B_DN of a band's first row */
     const int nb = (ns + NW - 1) / NW;
     for (int b = nb - 1; b >= 0; b--) {
@@ -503,12 +508,9 @@ __global__ __launch_bounds__(512) void k_map_bwd(ScoreCtx c, MapArgs m)
          * B_MM, S' at column c+1 (sp*) */
         float shmm = 0.0f, shmi = LP0, shdn = LP0, shs = LP0;
         float spmm = 0.0f, sps = LP0;
-        float sdn[R], fn[R], dxn[7];
+        float sdn[R], dxn[7];
 #pragma unroll
-        for (int k = 0; k < R; k++) {
-            const int64_t ix = lsi<R>(s, nsteps - 1, k, lane, nsteps);
-            sdn[k] = have ? Sd[ix] : LP0; fn[k] = have ? F[ix] : 0.0f;
-        }
+        for (int k = 0; k < R; k++) sdn[k] = have ? Sd[lsi<R>(s, nsteps - 1, k, lane, nsteps)] : LP0;
         bwd_trn(m.trans, ntr, db, nsteps - 1 - lane, Ld, dxn);
         const int TT = nsteps + 1 + 64 * (nwb - 1);
         for (int T = 0; T < TT; T++) {
@@ -516,17 +518,14 @@ __global__ __launch_bounds__(512) void k_map_bwd(ScoreCtx c, MapArgs m)
             const int t = nsteps - T + 64 * (nwb - 1 - w);
             if (have && t >= 0 && t < nsteps) {
                 const int cc = t - lane;
-                float sdc[R], fc[R], dx[7];
+                float sdc[R], dx[7];
 #pragma unroll
-                for (int k = 0; k < R; k++) { sdc[k] = sdn[k]; fc[k] = fn[k]; }
+                for (int k = 0; k < R; k++) sdc[k] = sdn[k];
 #pragma unroll
                 for (int j = 0; j < 7; j++) dx[j] = dxn[j];
                 if (t > 0) {
 #pragma unroll
-                    for (int k = 0; k < R; k++) {
-                        const int64_t ix = lsi<R>(s, t - 1, k, lane, nsteps);
-                        sdn[k] = Sd[ix]; fn[k] = F[ix];
-                    }
+                    for (int k = 0; k < R; k++) sdn[k] = Sd[lsi<R>(s, t - 1, k, lane, nsteps)];
                     bwd_trn(m.trans, ntr, db, cc - 1, Ld, dxn);
                 }
                 if (cc >= 0 && cc < Ld) {
@@ -542,7 +541,7 @@ __global__ __launch_bounds__(512) void k_map_bwd(ScoreCtx c, MapArgs m)
                         S5 n = init5();
                         if (valid[k]) {
                             n = bcell(dgmm, nmi, ndn, rt, qt[k], dx, dsc);
-                            F[ix] = fc[k] + n.mm;
+                            Bm[ix] = n.mm;
                         }
                         cur[k] = n; scur[k] = sk;
                         nmi = n.mi; ndn = n.dn;
@@ -607,7 +606,8 @@ __global__ __launch_bounds__(512) void k_map_dp(ScoreCtx c, MapArgs m)
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, NW = blockDim.x >> 6;
     const MapPair mp = m.pairs[pi];
     const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63, ns = m.ns;
-    const float* FB = m.F + mp.foff;
+    const float* Fm = m.F + mp.foff;
+    const float* Bm = m.B + mp.foff;
     char* bt = m.btck + mp.boff;
     float* vh = m.hand + mp.hoff + (size_t)ns * 5 * Ld + 3 * (size_t)Ld;   /* [Ld] a band's last row */
     const float logZ = m.stat[(size_t)pi * 4 + 1];
@@ -631,9 +631,14 @@ __global__ __launch_bounds__(512) void k_map_dp(ScoreCtx c, MapArgs m)
 #pragma unroll
         for (int k = 0; k < R; k++) V[k] = 0.0f;
         float U = 0.0f, D = 0.0f;
-        float fbn[R];
+        /* F and B of the next step, loaded one step ahead; their sum is the
+         * posterior's log numerator */
+        float fn_[R], bn_[R];
 #pragma unroll
-        for (int k = 0; k < R; k++) fbn[k] = have ? FB[lsi<R>(s, 0, k, lane, nsteps)] : 0.0f;
+        for (int k = 0; k < R; k++) {
+            const int64_t ix = lsi<R>(s, 0, k, lane, nsteps);
+            fn_[k] = have ? Fm[ix] : 0.0f; bn_[k] = have ? Bm[ix] : 0.0f;
+        }
         const int TT = nsteps + 1 + 64 * (nwb - 1);
         for (int T = 0; T < TT; T++) {
             const int t = T - 1 - 64 * w;
@@ -641,10 +646,13 @@ __global__ __launch_bounds__(512) void k_map_dp(ScoreCtx c, MapArgs m)
                 const int cc = t - lane;
                 float fbc[R];
 #pragma unroll
-                for (int k = 0; k < R; k++) fbc[k] = fbn[k];
+                for (int k = 0; k < R; k++) fbc[k] = fn_[k] + bn_[k];
                 if (t + 1 < nsteps) {
 #pragma unroll
-                    for (int k = 0; k < R; k++) fbn[k] = FB[lsi<R>(s, t + 1, k, lane, nsteps)];
+                    for (int k = 0; k < R; k++) {
+                        const int64_t ix = lsi<R>(s, t + 1, k, lane, nsteps);
+                        fn_[k] = Fm[ix]; bn_[k] = Bm[ix];
+                    }
                 }
                 if (cc >= 0 && cc < Ld) {
                     float prevk = D;
@@ -705,64 +713,51 @@ __global__ __launch_bounds__(512) void k_map_dp(ScoreCtx c, MapArgs m)
     }
 }
 
-/* ---- backtrace and drawing (FinalizeMAPDP_MAP), one thread per pair ---- */
+/* ---- backtrace and drawing (FinalizeMAPDP_MAP), one wave per pair ----
+ * c2_walk_draw over the MAP DP's codes, from the DP maximum */
 template <int R>
 __global__ __launch_bounds__(64) void k_map_final(ScoreCtx c, MapArgs m, int npairs)
 {
-    const int oi = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ __align__(16) char win[C2_DRAW_WIN];
+    const int oi = blockIdx.x;
     if (oi >= npairs) return;
     const int pi = m.order[oi];
     const MapPair mp = m.pairs[pi];
-    const int p = mp.pro, Ld = c.len[p], nsteps = Ld + 63;
-    const char* bt = m.btck + mp.boff;
+    const int p = mp.pro, Ld = c.len[p];
     int32_t* o = m.res + (size_t)pi * 8;
     const float best = __int_as_float(o[6]);
     const uint32_t be = (uint32_t)o[5];
-    int x = best > 0.0f ? (int)(be >> 16) : 0, y = best > 0.0f ? (int)(be & 0xFFFFu) : 0;
-    int n = 0, idts = 0, psts = 0, gaps = 0;
-    const int64_t col0 = c.dist[p];
+    const int x = best > 0.0f ? (int)(be >> 16) : 0, y = best > 0.0f ? (int)(be & 0xFFFFu) : 0;
+    MapCodes<R> codes;
+    codes.bt = m.btck + mp.boff; codes.nsteps = Ld + 63; codes.win = win;
     const int stride = (int)m.alnstride;
-    char* out = m.alns + (size_t)pi * 3 * stride;
-    char* tmp = m.alntmp + (size_t)pi * 3 * stride;
-    float w[20];
-    c2_pair_weights(c, p, w);
-    while (x >= 0 && y >= 0) {
-        const char b = bt[cell_lsi<R>(y, x, nsteps)];
-        if (b == 0) break;
-        char qa, ta, mid;
-        if (b == 2) { gaps++; qa = c.qaa[y]; ta = '-'; mid = ' '; y--; }
-        else if (b == 1) { gaps++; qa = '-'; ta = c.aa[col0 + x]; mid = ' '; x--; }
-        else {
-            qa = c.qaa[y]; ta = c.aa[col0 + x];
-            float ss;
-            if (qa == ta) { mid = qa; idts++; }
-            else if (c2_score_cell_w(c, y, col0 + x, w, &ss) > 0.0f) { mid = '+'; psts++; }
-            else mid = ' ';
-            x--; y--;
-        }
-        tmp[n] = qa; tmp[stride + n] = mid; tmp[2 * (size_t)stride + n] = ta; n++;
-    }
-    for (int k = 0; k < n; k++) {
-        out[k] = tmp[n - 1 - k];
-        out[stride + k] = tmp[stride + n - 1 - k];
-        out[2 * (size_t)stride + k] = tmp[2 * (size_t)stride + n - 1 - k];
-    }
-    out[n] = out[stride + n] = out[2 * (size_t)stride + n] = 0;
-    o[0] = n; o[1] = idts; o[2] = psts; o[3] = gaps;
-    o[4] = (int32_t)(((uint32_t)(x + 1) << 16) | (uint32_t)(y + 1));
+    c2_walk_draw(c, p, x, y, codes, m.alns + (size_t)pi * 3 * stride, m.alntmp + (size_t)pi * 3 * stride,
+                 stride, o);
 }
 
 template <int R>
-hipError_t launch_r(const ScoreCtx& c, const MapArgs& m, int npairs, hipStream_t stream)
+hipError_t launch_r(const ScoreCtx& c, const MapArgs& m, int npairs, hipStream_t stream,
+                    hipStream_t bstream, hipEvent_t fork, hipEvent_t join)
 {
     const int wg = 64 * m.nw;
+    hipError_t e;
     if (m.nsitems > 0)
         hipLaunchKernelGGL((k_map_score<R>), dim3((m.nsitems + 3) / 4), dim3(256), 0, stream, c, m);
+    if (bstream) {
+        if ((e = hipEventRecord(fork, stream)) != hipSuccess) return e;
+        if ((e = hipStreamWaitEvent(bstream, fork, 0)) != hipSuccess) return e;
+        hipLaunchKernelGGL((k_map_bwd<R>), dim3(npairs), dim3(wg), 0, bstream, c, m);
+        if ((e = hipEventRecord(join, bstream)) != hipSuccess) return e;
+    }
     hipLaunchKernelGGL((k_map_fwd<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
     hipLaunchKernelGGL((k_map_total<R>), dim3(npairs), dim3(256), 0, stream, c, m);
-    hipLaunchKernelGGL((k_map_bwd<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
+    if (bstream) {
+        if ((e = hipStreamWaitEvent(stream, join, 0)) != hipSuccess) return e;
+    } else {
+        hipLaunchKernelGGL((k_map_bwd<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
+    }
     hipLaunchKernelGGL((k_map_dp<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
-    hipLaunchKernelGGL((k_map_final<R>), dim3((npairs + 63) / 64), dim3(64), 0, stream, c, m, npairs);
+    hipLaunchKernelGGL((k_map_final<R>), dim3(npairs), dim3(64), 0, stream, c, m, npairs);
     return hipGetLastError();
 }
 
@@ -782,17 +777,19 @@ void c2_map_geometry(int Lq, int* R, int* ns, int* nw)
     *R = r; *ns = n; *nw = n < 8 ? n : 8;
 }
 
-hipError_t c2_launch_map(const ScoreCtx& c, const MapArgs& m, int npairs, hipStream_t stream)
+hipError_t c2_launch_map(const ScoreCtx& c, const MapArgs& m, int npairs, hipStream_t stream,
+                         hipStream_t bstream, hipEvent_t fork, hipEvent_t join)
 {
     if (npairs <= 0) return hipSuccess;
-    if (m.nw < 1 || m.nw > 8 || m.ns < 1 || !m.order || !m.sitems || m.nsitems < npairs * m.ns)
+    if (m.nw < 1 || m.nw > 8 || m.ns < 1 || !m.order || !m.sitems || m.nsitems < npairs * m.ns || !m.B)
         return hipErrorInvalidValue;
+    if (bstream && (!fork || !join)) return hipErrorInvalidValue;
     switch (m.R) {
-        case 1: return launch_r<1>(c, m, npairs, stream);
-        case 2: return launch_r<2>(c, m, npairs, stream);
-        case 3: return launch_r<3>(c, m, npairs, stream);
-        case 4: return launch_r<4>(c, m, npairs, stream);
-        case 5: return launch_r<5>(c, m, npairs, stream);
+        case 1: return launch_r<1>(c, m, npairs, stream, bstream, fork, join);
+        case 2: return launch_r<2>(c, m, npairs, stream, bstream, fork, join);
+        case 3: return launch_r<3>(c, m, npairs, stream, bstream, fork, join);
+        case 4: return launch_r<4>(c, m, npairs, stream, bstream, fork, join);
+        case 5: return launch_r<5>(c, m, npairs, stream, bstream, fork, join);
         default: return hipErrorInvalidValue;
     }
 }

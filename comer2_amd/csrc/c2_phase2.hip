@@ -22,7 +22,7 @@
  * the lanes and combined by a fixed xor butterfly, so the Newton/bisection
  * decisions are wave-uniform and the result does not depend on scheduling.
  */
-#include "c2_score.h"
+#include "c2_draw.h"
 
 namespace {
 
@@ -92,6 +92,27 @@ __global__ void k_finalize(ScoreCtx c, const int32_t* cand, int32_t ncand,
     r.end = ((uint32_t)xe << 16) | (uint32_t)ye;
     r.pad = 0;
     out[i] = r;
+}
+
+/* FinalizeDP_ALN (cudp/CuBatchDP_aln_final.cu:61-438): alignments of the
+ * MAPALN=0 path drawn from the phase-1 backtrace, one wave per pair;
+ * end coordinates are the phase-1 maximum's */
+__global__ __launch_bounds__(64) void k_aln_final(ScoreCtx c, AlnArgs a, int n)
+{
+    __shared__ __align__(16) uint32_t win[C2_DRAW_WIN / 4];
+    const int i = blockIdx.x;
+    if (i >= n) return;
+    const int p = a.pro[i];
+    const unsigned long long key = a.keys[p];
+    const uint32_t lo = 0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFull);
+    const int xe = key ? (int)(lo >> 16) : 0, ye = key ? (int)(lo & 0xFFFF) : 0;
+    P1Codes codes;
+    codes.bt = a.btck + a.btck_off[a.cand[i]]; codes.nsteps = c.len[p] + 63; codes.R = c.R; codes.win = win;
+    const int stride = (int)a.alnstride;
+    int32_t* o = a.res + (size_t)i * 8;
+    c2_walk_draw(c, p, xe, ye, codes, a.alns + (size_t)i * 3 * stride, a.alntmp + (size_t)i * 3 * stride,
+                 stride, o);
+    if (threadIdx.x == 0) o[5] = (int32_t)(((uint32_t)xe << 16) | (uint32_t)ye);
 }
 
 constexpr int NSC = 641;
@@ -582,6 +603,15 @@ hipError_t c2_launch_finalize(const ScoreCtx& c, const int32_t* cand, int32_t nc
 {
     if (ncand <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_finalize, dim3((ncand + 63) / 64), dim3(64), 0, stream, c, cand, ncand, keys, btck, btck_off, out);
+    return hipGetLastError();
+}
+
+hipError_t c2_launch_aln_final(const ScoreCtx& c, const AlnArgs& a, int n, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    if (!a.pro || !a.cand || !a.keys || !a.btck || !a.btck_off || !a.alns || !a.alntmp || !a.res)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_aln_final, dim3(n), dim3(64), 0, stream, c, a, n);
     return hipGetLastError();
 }
 

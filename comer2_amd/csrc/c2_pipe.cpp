@@ -451,6 +451,59 @@ int stage_stats(cbp_handle* h, QWork& w)
     return CBP_OK;
 }
 
+/* the drawn pairs' records and strings back to the host (pinned) */
+int drawn_results_copy(cbp_handle* h, QWork& w)
+{
+    const int nmap = (int)w.sel.size();
+    w.hmres.resize((size_t)8 * nmap);
+    /* the strings (NUL-padded to the stride, tens of MB for thousands of
+     * pairs) come back through pinned memory in one transfer */
+    const size_t nstr = (size_t)nmap * 3 * w.alnstride;
+    if (w.strs_cap < nstr) {
+        if (w.strs) { (void)hipHostFree(w.strs); w.strs = nullptr; w.strs_cap = 0; }
+        PCHK(hipHostMalloc((void**)&w.strs, nstr, hipHostMallocDefault));
+        w.strs_cap = nstr;
+    }
+    PCHK(hipMemcpyAsync(w.hmres.data(), w.mres.p, sizeof(int32_t) * w.hmres.size(), hipMemcpyDeviceToHost, w.st));
+    PCHK(hipMemcpyAsync(w.strs, w.malns.p, nstr, hipMemcpyDeviceToHost, w.st));
+    return CBP_OK;
+}
+
+/* 4 (MAPALN=0). the selected pairs drawn from their phase-1 backtrace
+ * (FinalizeDP_ALN, cudp/CuBatchDP_aln_final.cu:61-438, launched by
+ * CuBatchDP::FinalizeALNDynProgDevice, cudp/CuBatchDP.cu:432-528) */
+int stage_aln(cbp_handle* h, QWork& w)
+{
+    const cbp_profiles& ch = h->chunk.v;
+    const int nsel = (int)w.sel.size();
+    w.apro.resize(nsel); w.acand.resize(nsel);
+    int maxld = 0;
+    for (int k = 0; k < nsel; k++) {
+        w.apro[k] = w.pass[w.sel[k]];
+        w.acand[k] = w.passfin[w.sel[k]];
+        if (ch.len[w.apro[k]] > maxld) maxld = ch.len[w.apro[k]];
+    }
+    w.alnstride = (int64_t)w.dq->hp.Lq + maxld + 1;
+    int rc;
+    if ((rc = c2_dupload(h, w.dapro, w.apro.data(), nsel, w.st))) return rc;
+    if ((rc = c2_dupload(h, w.dacand, w.acand.data(), nsel, w.st))) return rc;
+    const size_t alnwords = ((size_t)nsel * 3 * w.alnstride + 3) / 4;
+    if ((rc = c2_dalloc(h, w.malns, alnwords * 4))) return rc;
+    if ((rc = c2_dalloc(h, w.malntmp, (size_t)nsel * 3 * w.alnstride))) return rc;
+    if ((rc = c2_dalloc(h, w.mres, sizeof(int32_t) * 8 * (size_t)nsel))) return rc;
+    ScoreCtx sc;
+    c2_fill_scorectx(h, w.dq, &sc);
+    AlnArgs a;
+    a.pro = w.dapro.as<int32_t>(); a.cand = w.dacand.as<int32_t>(); a.keys = w.dk;
+    a.btck = w.btck.as<uint32_t>(); a.btck_off = w.doff.as<int64_t>();
+    a.alns = w.malns.as<char>(); a.alntmp = w.malntmp.as<char>(); a.alnstride = w.alnstride;
+    a.res = w.mres.as<int32_t>();
+    PCHK(c2_launch_zero_u32(w.malns.as<uint32_t>(), (int64_t)alnwords, w.st));
+    PCHK(c2_launch_zero_u32(w.mres.as<uint32_t>(), (int64_t)8 * nsel, w.st));
+    PCHK(c2_launch_aln_final(sc, a, nsel, w.st));
+    return drawn_results_copy(h, w);
+}
+
 /* 4. statistics into the records, order, MAP selection and launch */
 int stage_map(cbp_handle* h, QWork& w, bool split)
 {
@@ -474,7 +527,6 @@ int stage_map(cbp_handle* h, QWork& w, bool split)
     });
     w.sel.clear();
     w.alnstride = 0;
-    if (!h->opt.mapaln) return CBP_OK;
     for (int k = 0; k < npass; k++) {
         const int i = w.order[k];
         if (w.aln[(size_t)i * CBP_NFIELDS + CBP_F_EVALUE] <= w.logevthld) w.sel.push_back(i);
@@ -486,6 +538,7 @@ int stage_map(cbp_handle* h, QWork& w, bool split)
     }
     const int nmap = (int)w.sel.size();
     if (nmap == 0) return CBP_OK;
+    if (!h->opt.mapaln) return stage_aln(h, w);
     int mR = 1, mns = 1, mnw = 1;
     c2_map_geometry(dq->hp.Lq, &mR, &mns, &mnw);
     w.mp.resize(nmap);
@@ -531,6 +584,7 @@ int stage_map(cbp_handle* h, QWork& w, bool split)
     if ((rc = c2_dupload(h, w.dmorder, w.morder.data(), nmap, w.st))) return rc;
     if ((rc = c2_dupload(h, w.dmsitems, w.msitems.data(), w.msitems.size(), w.st))) return rc;
     if ((rc = c2_dalloc(h, w.mF, sizeof(float) * (size_t)foff))) return rc;
+    if ((rc = c2_dalloc(h, w.mB, sizeof(float) * (size_t)foff))) return rc;
     if ((rc = c2_dalloc(h, w.mSd, sizeof(float) * (size_t)foff))) return rc;
     if ((rc = c2_dalloc(h, w.mhand, sizeof(float) * (size_t)hoff))) return rc;
     if ((rc = c2_dalloc(h, w.mbt, (size_t)boff))) return rc;
@@ -546,7 +600,7 @@ int stage_map(cbp_handle* h, QWork& w, bool split)
     ma.pairs = w.mpairs.as<MapPair>(); ma.trans = h->dchunk.trans.as<float>();
     ma.qtr = dq->qtr.as<float>(); ma.qtrb = dq->qtrb.as<float>();
     ma.mleft = dq->mleft.as<float>(); ma.mdiag = dq->mdiag.as<float>();
-    ma.F = w.mF.as<float>(); ma.Sd = w.mSd.as<float>(); ma.hand = w.mhand.as<float>();
+    ma.F = w.mF.as<float>(); ma.B = w.mB.as<float>(); ma.Sd = w.mSd.as<float>(); ma.hand = w.mhand.as<float>();
     ma.btck = w.mbt.as<char>(); ma.stat = w.mstat.as<float>();
     ma.alns = w.malns.as<char>(); ma.alntmp = w.malntmp.as<char>(); ma.alnstride = w.alnstride;
     ma.res = w.mres.as<int32_t>(); ma.minpp = h->opt.minpp;
@@ -557,7 +611,16 @@ int stage_map(cbp_handle* h, QWork& w, bool split)
     /* every MAP pass is as long as its longest pair's column sweep; for a
      * query on its own, the pairs longer than half the longest run their
      * passes on a second stream, so the other pairs' passes fill the device
-     * meanwhile (per-pair outputs go to the same slots either way) */
+     * meanwhile (per-pair outputs go to the same slots either way), and each
+     * group's backward pass runs on a stream of its own beside its forward
+     * pass */
+    if (split) {
+        for (int g = 0; g < 2; g++)
+            if (!h->mfork[g]) {
+                PCHK(hipEventCreateWithFlags(&h->mfork[g], hipEventDisableTiming));
+                PCHK(hipEventCreateWithFlags(&h->mjoin[g], hipEventDisableTiming));
+            }
+    }
     int nA = 0;
     if (split && nmap >= 2) {
         const int lmax = ch.len[w.mp[w.morder[0]].pro];
@@ -576,31 +639,32 @@ int stage_map(cbp_handle* h, QWork& w, bool split)
         mB.nsitems = ma.nsitems - (int32_t)itA;
         PCHK(hipEventRecord(h->gfork, w.st));
         PCHK(hipStreamWaitEvent(sA, h->gfork, 0));
-        PCHK(c2_launch_map(sc, mA, nA, sA));
-        PCHK(c2_launch_map(sc, mB, nmap - nA, w.st));
+        PCHK(c2_launch_map(sc, mA, nA, sA, h->xstream[1], h->mfork[0], h->mjoin[0]));
+        PCHK(c2_launch_map(sc, mB, nmap - nA, w.st, h->xstream[2], h->mfork[1], h->mjoin[1]));
         PCHK(hipEventRecord(h->gjoin[0], sA));
         PCHK(hipStreamWaitEvent(w.st, h->gjoin[0], 0));
+    } else if (split) {
+        PCHK(c2_launch_map(sc, ma, nmap, w.st, h->xstream[1], h->mfork[0], h->mjoin[0]));
     } else {
-        PCHK(c2_launch_map(sc, ma, nmap, w.st));
+        PCHK(c2_launch_map(sc, ma, nmap, w.st, nullptr, nullptr, nullptr));
     }
-    w.hmres.resize((size_t)8 * nmap);
-    /* the strings (NUL-padded to the stride, tens of MB for thousands of
-     * pairs) come back through pinned memory in one transfer */
-    const size_t nstr = (size_t)nmap * 3 * w.alnstride;
-    if (w.strs_cap < nstr) {
-        if (w.strs) { (void)hipHostFree(w.strs); w.strs = nullptr; w.strs_cap = 0; }
-        PCHK(hipHostMalloc((void**)&w.strs, nstr, hipHostMallocDefault));
-        w.strs_cap = nstr;
-    }
-    PCHK(hipMemcpyAsync(w.hmres.data(), w.mres.p, sizeof(int32_t) * w.hmres.size(), hipMemcpyDeviceToHost, w.st));
-    PCHK(hipMemcpyAsync(w.strs, w.malns.p, nstr, hipMemcpyDeviceToHost, w.st));
-    return CBP_OK;
+    return drawn_results_copy(h, w);
 }
 
 /* 5. MAP results into the records; the caller's view */
 void stage_results(cbp_handle* h, QWork& w, cbp_results* out)
 {
     const int nmap = (int)w.sel.size();
+    /* the drawn rows' phase-1 fields (positives, coordinates) before the
+     * drawing replaces them: a merge over DB chunks restores them for rows
+     * that fall beyond the whole DB's cap on drawn pairs */
+    w.p1save.resize((size_t)3 * nmap);
+    for (int k = 0; k < nmap; k++) {
+        const float* r = &w.aln[(size_t)w.sel[k] * CBP_NFIELDS];
+        w.p1save[3 * k] = r[CBP_F_PSTVS];
+        w.p1save[3 * k + 1] = r[CBP_F_BEGCOORDS];
+        w.p1save[3 * k + 2] = r[CBP_F_ENDCOORDS];
+    }
     for (int k = 0; k < nmap; k++) {
         float* r = &w.aln[(size_t)w.sel[k] * CBP_NFIELDS];
         const int32_t* o = &w.hmres[(size_t)8 * k];

@@ -57,6 +57,8 @@ def lib():
     L.cbp_process.argtypes = [H, C.c_int32, C.c_float, C.c_float, C.POINTER(Results)]
     L.cbp_process_batch.argtypes = [H, C.c_void_p, C.c_int32, C.POINTER(Results)]
     L.cbp_score_block.argtypes = [H, C.c_int32, C.c_int32, C.POINTER(C.c_float), C.POINTER(C.c_float)]
+    L.cbp_search_db.argtypes = [H, C.POINTER(Profiles), C.c_uint64, C.c_void_p, C.c_int32, C.POINTER(Results)]
+    L.cbp_last_search_chunks.argtypes = [H, C.POINTER(C.c_int32)]
     V = C.c_void_p
     L.cbp_sdp_prepare.argtypes = [H, C.POINTER(_ffi.SdpPair), C.c_int32, C.c_int32, C.POINTER(C.c_int64)]
     L.cbp_sdp_scores.argtypes = [H, V, V]
@@ -159,6 +161,24 @@ class CuBatchProcessing:
         outs = (Results * max(1, q.size))()
         self._check(self.L.cbp_process_batch(self.h, q.ctypes.data_as(C.c_void_p), q.size, outs))
         return [results_to_dict(outs[i]) for i in range(q.size)]
+
+    def SearchDB(self, dbset, qidxs, chunk_pos=0):
+        """All listed queries against a whole DB in host memory, uploaded in
+        chunks of at most chunk_pos positions (0: from the chunk limits) while
+        the previous chunk is searched; per query the chunks' hit lists are
+        merged (one result dict per query, rows in whole-DB profile order).
+        The uploaded chunk, if any, is released."""
+        q = np.ascontiguousarray(qidxs, np.int32)
+        outs = (Results * max(1, q.size))()
+        self._check(self.L.cbp_search_db(self.h, C.byref(dbset.view), int(chunk_pos),
+                                         q.ctypes.data_as(C.c_void_p), q.size, outs))
+        self._chunk = None
+        return [results_to_dict(outs[i]) for i in range(q.size)]
+
+    def last_search_chunks(self):
+        n = C.c_int32()
+        self._check(self.L.cbp_last_search_chunks(self.h, C.byref(n)))
+        return n.value
 
     def score_block(self, qidx, pronr, want_mod=True):
         """FP32 score matrix (and SS component) of one pair, [Lq][Ldb]."""

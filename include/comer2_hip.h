@@ -82,7 +82,9 @@ typedef struct {
     int32_t      npassed;    /* pairs passing phase 1 (DP + correlation) */
     int64_t      nposits;    /* sum of passed profile lengths */
     int32_t      maxlen;     /* longest passed profile */
-    int32_t      nmap;       /* pairs realigned by MAP (first nmap of order) */
+    int32_t      nmap;       /* pairs drawn: realigned by MAP (MAPALN=1) or drawn
+                                from the phase-1 backtrace (MAPALN=0), the
+                                selected pairs in order */
     const float* alndata;    /* [npassed][CBP_NFIELDS] in profile order */
     const int32_t* order;    /* [npassed] indices into alndata sorted by (log E, profile index) */
     const char*  alns;       /* alignment strings: per realigned pair, 3 rows
@@ -177,6 +179,28 @@ int  cbp_process(cbp_handle* h, int32_t qidx, float scorethld, float logevthld,
  * outs[i] stays valid until the next cbp_process* call on the handle */
 int  cbp_process_batch(cbp_handle* h, const int32_t* qidx, int32_t nq,
                        cbp_results* outs /* [nq] */);
+
+/* whole-database search of the listed queries in chunks (the reference's
+ * chunk loop: JobDispatcher reads the DB chunk by chunk,
+ * cualn/JobDispatcher.cpp:1103-1134, with TdDataReader's read-ahead,
+ * cupro/TdDataReader.cpp:575-692; DevCommThread searches every query on each
+ * chunk, cualn/DevCommThread.cpp:296-356; AlnWriter::MergeResults merges a
+ * query's parts, cualn/AlnWriter.cpp:218-267).  db is the whole DB in host
+ * memory (borrowed for the call); it is cut into chunks of whole profiles of
+ * at most chunk_pos positions (0: half of cbp_chunk_limits' maximum for the
+ * longest listed query, two chunks being resident); a copy thread uploads
+ * chunk k+1 through pinned staging buffers while chunk k is searched; the
+ * thresholds and statistics use the whole DB's size (cbp_set_db_details, or
+ * db's own size when unset).  outs[i] is query qidx[i]'s result over the
+ * whole DB, equal to a search of the DB as one chunk: rows in whole-DB
+ * profile order, order by (log E, profile index), the first max_map_pairs
+ * selected pairs drawn.  The handle's own chunk (cbp_upload_chunk) is
+ * released for searching: upload a chunk again before cbp_process.
+ * outs stay valid until the next cbp_search_db call on the handle. */
+int  cbp_search_db(cbp_handle* h, const cbp_profiles* db, uint64_t chunk_pos,
+                   const int32_t* qidx, int32_t nq, cbp_results* outs /* [nq] */);
+/* number of chunks of the last cbp_search_db call */
+int  cbp_last_search_chunks(const cbp_handle* h, int32_t* nchunks);
 
 /* ---- stage entry points (exposed for parity tests and profiling) ---- */
 

@@ -74,6 +74,18 @@ or_fin or_finalize(int Lq, int Ldb, const float* S, const char* btck, or_dpmax m
 /* per-cell backtrack codes */
 enum { OR_STOP = 0, OR_LEFT = 1, OR_UP = 2, OR_DIAG = 3 };
 
+/* FinalizeDP_ALN (cudp/CuBatchDP_aln_final.cu:206-345): the MAPALN=0 path's
+ * alignment drawn from the phase-1 backtrace, starting at the phase-1
+ * maximum; S raw [Lq][Ldb] (middle '+' where S > 0); qaa/daa the residues;
+ * aln [3][Lq+Ldb+1] (query, middle, target, NUL-terminated) or NULL */
+typedef struct {
+    int32_t alnlen, idts, psts, gaps;
+    int32_t begx, begy;       /* (x+1, y+1) after the backtrace */
+    int32_t endx, endy;       /* the phase-1 maximum */
+} or_aln;
+or_aln or_aln_final(int Lq, int Ldb, const float* S, const char* btck, or_dpmax m,
+                    const char* qaa, const char* daa, char* aln);
+
 /* score threshold (JobDispatcher.cpp:809-860) */
 void or_score_threshold(const cbp_options* o, uint64_t qlen, uint64_t dblen,
                         float* scorethld, float* logevthld);

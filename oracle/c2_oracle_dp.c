@@ -497,6 +497,51 @@ or_fin or_finalize(int Lq, int Ld, const float* S, const char* btck, or_dpmax m)
     return f;
 }
 
+/* FinalizeDP_ALN (cudp/CuBatchDP_aln_final.cu:206-345): the walk of
+ * :246-306 (STOP at the matrix edge or a STOP code; UP y--, LEFT x--, DIAG
+ * both), the middle symbol of :299-305, the reversed write of :314-328 and
+ * the beginning coordinates (x+1, y+1) of :336 */
+or_aln or_aln_final(int Lq, int Ld, const float* S, const char* btck, or_dpmax m,
+                    const char* qaa, const char* daa, char* aln)
+{
+    or_aln r;
+    memset(&r, 0, sizeof r);
+    int x = (m.maxsc > 0.0f) ? m.x : 0, y = (m.maxsc > 0.0f) ? m.y : 0;
+    r.endx = x; r.endy = y;
+    const int stride = Lq + Ld + 1;
+    char* buf = (char*)malloc(3 * (size_t)stride);
+    int n = 0;
+    for (;;) {
+        if (x < 0 || y < 0) break;
+        const size_t qpos = (size_t)y * Ld + x;
+        const char b = btck[qpos];
+        if (b == OR_STOP) break;
+        char qa, ta, mid;
+        if (b == OR_UP) { r.gaps++; qa = qaa[y]; ta = '-'; mid = ' '; y--; }
+        else if (b == OR_LEFT) { r.gaps++; qa = '-'; ta = daa[x]; mid = ' '; x--; }
+        else {
+            qa = qaa[y]; ta = daa[x];
+            if (qa == ta) { mid = qa; r.idts++; }
+            else if (S[qpos] > 0.0f) { mid = '+'; r.psts++; }
+            else mid = ' ';
+            x--; y--;
+        }
+        buf[n] = qa; buf[stride + n] = mid; buf[2 * (size_t)stride + n] = ta; n++;
+    }
+    r.alnlen = n;
+    r.begx = x + 1; r.begy = y + 1;
+    if (aln) {
+        for (int k = 0; k < n; k++) {
+            aln[k] = buf[n - 1 - k];
+            aln[stride + k] = buf[stride + n - 1 - k];
+            aln[2 * (size_t)stride + k] = buf[2 * (size_t)stride + n - 1 - k];
+        }
+        aln[n] = aln[stride + n] = aln[2 * (size_t)stride + n] = 0;
+    }
+    free(buf);
+    return r;
+}
+
 /* JobDispatcher::CalculateScoreThreshold (cualn/JobDispatcher.cpp:809-860) */
 void or_score_threshold(const cbp_options* o, uint64_t qlen, uint64_t dblen,
                         float* scorethld, float* logevthld)

@@ -25,6 +25,10 @@ class Fin(C.Structure):
                 ("begy", C.c_int32), ("endx", C.c_int32), ("endy", C.c_int32)]
 
 
+class Aln(C.Structure):
+    _fields_ = [(n, C.c_int32) for n in ("alnlen", "idts", "psts", "gaps", "begx", "begy", "endx", "endy")]
+
+
 _lib = None
 
 
@@ -50,6 +54,8 @@ def lib():
         L.or_finalize.argtypes = [C.c_int, C.c_int, fp, cp, DpMax]
         L.or_finalize.restype = Fin
         L.or_mask_corners_f32.argtypes = [C.c_int, C.c_int, fp]
+        L.or_aln_final.argtypes = [C.c_int, C.c_int, fp, cp, DpMax, C.c_char_p, C.c_char_p, C.c_void_p]
+        L.or_aln_final.restype = Aln
         L.or_score_threshold.argtypes = [O, C.c_uint64, C.c_uint64,
                                          C.POINTER(C.c_float), C.POINTER(C.c_float)]
         _lib = L
@@ -102,6 +108,13 @@ class Oracle:
     def finalize(self, S, bt, m):
         Lq, Ld = S.shape
         return self.L.or_finalize(Lq, Ld, np.ascontiguousarray(S), bt, m)
+
+    def aln_final(self, S, bt, m, qaa, daa):
+        """MAPALN=0 drawing of a pair from its phase-1 backtrace: (record, (query, middle, target))"""
+        Lq, Ld = S.shape
+        buf = np.zeros((3, Lq + Ld + 1), np.uint8)
+        r = self.L.or_aln_final(Lq, Ld, np.ascontiguousarray(S), bt, m, qaa, daa, buf.ctypes.data)
+        return r, tuple(bytes(buf[j]).split(b"\0")[0].decode() for j in range(3))
 
     def colm1(self, q, qp):
         Lq = q.len[qp]

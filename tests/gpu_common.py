@@ -43,7 +43,11 @@ def _pair_expect(orc, q, qp, d, dp, thld, ss, xuninf):
     cnt = orc.histogram(S, qaa, daa, xuninf)
     st = orc.statistics(cnt, q.view.len[qp], d.view.len[dp], q.view.eno[qp], d.view.eno[dp],
                         f.score, float(f.psts), np.float32(ss))
-    return dict(dp=dp, fin=f, st=st)
+    r = dict(dp=dp, fin=f, st=st)
+    if not orc.o.mapaln:
+        # MAPALN=0: the alignment drawn from the phase-1 backtrace (FinalizeDP_ALN)
+        r["aln"] = orc.aln_final(S, bt, m, qaa, daa)
+    return r
 
 
 def search_expect(orc, q, qp, d, thld, logev, dblen, nseqs, cap, xuninf=1, threads=16):
@@ -63,9 +67,15 @@ def search_expect(orc, q, qp, d, thld, logev, dblen, nseqs, cap, xuninf=1, threa
 
     def one_map(i):
         r = rows[i]
+        if "aln" in r:
+            a, strs = r["aln"]
+            return dict(alnlen=a.alnlen, idts=a.idts, psts=a.psts, gaps=a.gaps,
+                        beg=(a.begx << 16) | a.begy, end=(a.endx << 16) | a.endy), strs
         S, M = orc.score_f32(q.view, qp, d.view, r["dp"])
         mr, _, _, _, aln = orc.map(q.view, qp, d.view, r["dp"], S, M, np.float32(r["st"].EpA))
-        return mr, tuple(bytes(aln[j]).split(b"\0")[0].decode() for j in range(3))
+        return (dict(alnlen=mr.alnlen, idts=mr.idts, psts=mr.psts, gaps=mr.gaps,
+                     beg=(mr.begx << 16) | mr.begy, end=(mr.x << 16) | mr.y),
+                tuple(bytes(aln[j]).split(b"\0")[0].decode() for j in range(3)))
 
     with ThreadPoolExecutor(threads) as ex:
         maps = list(ex.map(one_map, sel))
@@ -98,13 +108,13 @@ def assert_search_equal(res, exp, ctx, stat_rtol=1e-6):
             got = float(res[name][k])
             assert abs(got - val) <= stat_rtol * max(1.0, abs(val)), (c, name, got, val)
     for j, (i, (mr, strs)) in enumerate(zip(exp["sel"], exp["maps"])):
-        c = ctx + ("map", i, rows[i]["dp"])
-        assert res["alnlen"][i] == mr.alnlen, c
-        assert res["idnts"][i] == mr.idts, c
-        assert res["psts"][i] == mr.psts, c
-        assert res["ngaps"][i] == mr.gaps, c
-        assert res["beg"][i] == ((mr.begx << 16) | mr.begy), c
-        assert res["end"][i] == ((mr.x << 16) | mr.y), c
+        c = ctx + ("drawn", i, rows[i]["dp"])
+        assert res["alnlen"][i] == mr["alnlen"], c
+        assert res["idnts"][i] == mr["idts"], c
+        assert res["psts"][i] == mr["psts"], c
+        assert res["ngaps"][i] == mr["gaps"], c
+        assert res["beg"][i] == mr["beg"], c
+        assert res["end"][i] == mr["end"], c
         assert res["alns"][j] == strs, c
 
 

@@ -16,8 +16,13 @@ oracle finishes in seconds.
 
 Tolerances: hit lists, coordinates, records and strings exact; statistics
 1e-6 relative (FP64 on both sides); soft DP log Z 1e-5 relative,
-posteriors 1e-4 relative above 1e-2 (1e-6 absolute below), dS 1e-4
-absolute, dtheta 1e-3 relative, gradients vs central differences 1e-3.
+posteriors 1e-4 relative above 1e-2 (1e-6 absolute below) for the checkpointed
+vs all-cells comparison and 2e-3 relative above 1e-2 (2e-5 absolute below)
+vs the FP64 oracle on the long pairs (FP32 rounding of log-space states of up to ~200 nats, ulp 1.5e-5,
+accumulated over up to 7,000 anti-diagonal steps: log Z is 1.2e-3 off in
+absolute terms on a 2,000 x 5,000 pair, and every posterior inherits that as
+relative error), dS 1e-4 absolute, dtheta 1e-3 relative, gradients vs
+central differences 1e-3.
 """
 import numpy as np
 import pytest
@@ -125,9 +130,9 @@ def test_x_residues_match_oracle(xuninf):
         assert any("X" in s[0] or "X" in s[2] for s in res["alns"])
 
 
-def _post_close(got, want):
-    """posteriors: 1e-4 relative above 1e-2, 1e-6 absolute below"""
-    tol = np.where(want >= 1e-2, 1e-4 * want, 1e-6)
+def _post_close(got, want, rtol=1e-4):
+    """posteriors: rtol relative above 1e-2, rtol * 1e-2 absolute below"""
+    tol = rtol * np.maximum(want, 1e-2)
     bad = np.abs(got - want) > tol
     return not bad.any(), float(np.max(np.abs(got - want) / np.maximum(want, 1e-2)))
 
@@ -163,6 +168,8 @@ def test_c4_long_pairs_checkpointed():
     # checkpointed vs all cells kept
     assert np.allclose(lz, out[0][1], rtol=1e-6, atol=0)
     assert np.max(np.abs(dS - out[0][2])) <= 1e-6
+    ok, worst = _post_close(post, out[0][4])
+    assert ok, ("checkpointed vs all cells", worst)
     assert np.allclose(dth, out[0][3], rtol=1e-5, atol=1e-5)
     # vs the FP64 oracle
     thd = th.cpu().numpy().astype(np.float64)
@@ -175,7 +182,7 @@ def test_c4_long_pairs_checkpointed():
         ctx = (qp, dp, L1, L2)
         assert abs(lz[k] - z) <= 1e-5 * abs(z), ctx
         assert np.max(np.abs(dS[off:off + n] - eS.ravel())) < 1e-4, ctx
-        ok, worst = _post_close(post[off:off + n], epost.ravel())
+        ok, worst = _post_close(post[off:off + n], epost.ravel(), rtol=2e-3)
         print(f"C4 pair {ctx}: logZ {lz[k]:.4f} vs {z:.4f}, posterior worst rel {worst:.2e}", flush=True)
         assert ok, (ctx, worst)
         assert np.all(np.abs(dth[k] - eth) <= 1e-3 * np.maximum(1.0, np.abs(eth))), (ctx, dth[k], eth)
@@ -198,3 +205,36 @@ def test_c4_long_pairs_checkpointed():
                 fd = (f(Sp, thd) - f(Sm, thd)) / (2 * h)
                 assert abs(dS[off + c] - fd) <= 1e-3, (i, j, dS[off + c], fd)
         off += n
+
+
+def test_mapaln0_phase1_drawing_matches_oracle():
+    """MAPALN=0: the selected pairs are drawn from their phase-1 backtrace
+    (FinalizeDP_ALN, cudp/CuBatchDP_aln_final.cu:61-438) instead of being
+    realigned, and the score shift is 0.0003 (CuBatchScoreMatrix_Init.cuh:14-15);
+    one-query calls and one batch call against the oracle.  Query lengths up
+    to 700 give two-stripe phase-1 codes."""
+    q = SynthDB(1602, 3, 300, 0.5, 100, 700)
+    d = SynthDB(1603, 120, 200, 0.668, 30, 1500)
+    d.plant(q, 1604, 30, 3.0, 60.0)
+    opts = dict(mapaln=0, max_map_pairs=0)
+    orc = oracle(opts)
+    cbp = processing(opts)
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    ql = q.lengths()
+    exps = []
+    drawn = 0
+    for qp in range(q.nprof):
+        print(f"MAPALN=0: query {qp} len {int(ql[qp])}", flush=True)
+        res = cbp.ProcessScoreMatrix(qp, 6.0, float(np.log(1e-2)))
+        exp = search_expect(orc, q, qp, d, 6.0, float(np.log(1e-2)), d.npos, d.nprof, 0)
+        assert_search_equal(res, exp, ("mapaln0", qp))
+        drawn += len(exp["sel"])
+        exps.append(exp)
+    assert drawn >= 10
+    batch = cbp.ProcessBatch(list(range(q.nprof)))
+    for qp, res in enumerate(batch):
+        thld, logev = cbp.score_threshold(int(ql[qp]), d.npos)
+        exp = search_expect(orc, q, qp, d, thld, logev, d.npos, d.nprof, 0)
+        assert_search_equal(res, exp, ("mapaln0 batch", qp))
