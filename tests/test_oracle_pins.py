@@ -449,3 +449,55 @@ def test_reference_maximum_misses_only_the_derived_cells(orc, qlen):
             differ += r1.maxsc != r2.maxsc
     if qlen <= 50:
         assert differ >= 1      # the deviation is real
+
+
+def test_aln_final_draws_a_planted_path(orc):
+    """MAPALN=0 drawing (or_aln_final, FinalizeDP_ALN) of a hand-made
+    phase-1 path whose residues are chosen so that every count follows by
+    construction: on diagonal cells the two residues agree where k % 3 == 0
+    (middle = the residue), differ with a positive score where k % 3 == 1
+    ('+'), and differ with a non-positive score otherwise (' '); up moves
+    put a query residue over '-', left moves '-' over a DB residue.  The
+    strings read start first, the beginning is one past where the walk
+    stops and the end is the maximum."""
+    moves = []
+    for k in range(40):
+        moves.append("D")
+        if k % 9 == 4:
+            moves.append("U")
+        if k % 13 == 6:
+            moves += ["L", "L"]
+    nd = moves.count("D")
+    vals = []
+    for k in range(nd):
+        vals.append(1.5 if k % 3 == 1 else (-0.5 if k % 3 == 2 else 0.25))
+    S, bt, m, beg, end = _planted_path(moves, vals, seed=3)
+    Lq, Ld = S.shape
+    letters = b"ACDEFGHIKLMNPQRSTVWY"
+    rng = np.random.default_rng(5)
+    qaa = bytearray(letters[i] for i in rng.integers(0, 20, Lq))
+    daa = bytearray(letters[i] for i in rng.integers(0, 20, Ld))
+    # walk the planted path to set residues and the expected strings
+    x, y, k = m.x, m.y, 0
+    top, mid, bot = [], [], []
+    for mv in moves:
+        if mv == "D":
+            if k % 3 == 0:
+                daa[x] = qaa[y]
+            elif daa[x] == qaa[y]:
+                daa[x] = letters[(letters.index(qaa[y]) + 1) % 20]
+            q_, d_ = chr(qaa[y]), chr(daa[x])
+            top.append(q_); bot.append(d_)
+            mid.append(q_ if k % 3 == 0 else ("+" if k % 3 == 1 else " "))
+            k += 1; x -= 1; y -= 1
+        elif mv == "U":
+            top.append(chr(qaa[y])); mid.append(" "); bot.append("-"); y -= 1
+        else:
+            top.append("-"); mid.append(" "); bot.append(chr(daa[x])); x -= 1
+    r, strs = orc.aln_final(S, bt, m, bytes(qaa), bytes(daa))
+    assert strs == ("".join(top[::-1]), "".join(mid[::-1]), "".join(bot[::-1]))
+    assert r.alnlen == len(moves)
+    assert r.idts == sum(1 for j in range(nd) if j % 3 == 0)
+    assert r.psts == sum(1 for j in range(nd) if j % 3 == 1)
+    assert r.gaps == moves.count("U") + moves.count("L")
+    assert (r.begx, r.begy) == beg and (r.endx, r.endy) == end
