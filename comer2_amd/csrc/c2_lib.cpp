@@ -92,6 +92,9 @@ int cbp_create(int device, size_t mem_budget, int area, cbp_handle** out)
         delete h;
         return CBP_ERR_HIP;
     }
+    /* the allocation-free entry point's scratch array: its address on this
+     * device (a failed lookup is reported by that entry point) */
+    (void)c2_noalloc_init(device);
     *out = h;
     return CBP_OK;
 }

@@ -255,6 +255,9 @@ inline int c2_dupload(cbp_handle* h, DBuf& b, const T* src, size_t count, hipStr
 DevQuery* c2_get_query(cbp_handle* h, int qidx, int* rc);
 int c2_append_query(cbp_handle* h, const cbp_profiles* one, int32_t* qidx);
 void c2_drop_graphs(cbp_handle* h);
+/* device address of the allocation-free entry point's scratch array on the
+ * current device, looked up once per device */
+hipError_t c2_noalloc_init(int device);
 int c2_fill_p1args(cbp_handle* h, DevQuery* dq, P1Args* a);
 void c2_fill_scorectx(cbp_handle* h, DevQuery* dq, ScoreCtx* c);
 uint64_t c2_length_adjustment(const cbp_options& o, uint64_t m, uint64_t n, uint64_t N, uint64_t* delta);

@@ -215,6 +215,19 @@ int  cbp_phase1_keys(cbp_handle* h, int32_t qidx, uint64_t* keys /* [nprof] */);
  * cusco/CuBatchScoreMatrix.cu:118-377) */
 int  cbp_score_block(cbp_handle* h, int32_t qidx, int32_t pronr, float* S, float* mod);
 
+/* cbp_phase1_keys without any allocation (no reference counterpart; for
+ * callers that cannot allocate device memory: code under HIP graph capture,
+ * tight serving loops, harnesses that forbid it).  Every device buffer the
+ * call writes besides the uploaded chunk and query set -- the maxima, the
+ * score weights, the stripe hand-off of multi-stripe queries and the
+ * query's phase-1 tables -- lives in one static device array of 64 MiB that
+ * each GPU the library runs on holds until the process ends; a query and
+ * chunk that need more (88 B per profile, 20 B per position for queries of
+ * more than one stripe of 64*R rows, about 350 B per query row) are refused
+ * with CBP_ERR_ARG before any launch.  The array is shared by every call on
+ * one GPU: calls that overlap on one GPU (two threads or two handles) race */
+int  cbp_phase1_keys_noalloc(cbp_handle* h, int32_t qidx, uint64_t* keys /* [nprof] */);
+
 /* tuning of the phase-1 launch geometry: target DB columns per wave segment
  * (>= 64), maximum query rows per lane (1..5) and waves per workgroup
  * (1, 2, 4, 8); no reference counterpart (the reference fixes 32x32 blocks,
