@@ -33,6 +33,12 @@ device work queue (persistent workgroups, tiles binned by rows per lane),
 and the same with phase 1 replayed from a HIP graph; hit lists compared bit
 for bit.
 
+The stream object reports the same searches with the DB in host memory,
+uploaded in chunks of about 12.6 M positions (4 chunks) through pinned
+staging buffers while the previous chunk is searched (cbp_search_db), the
+upload included in the time: the one-query search and the 64-query batch,
+hit lists compared with the resident-DB runs.
+
 The softdp object reports the differentiable soft DP: forward-only and
 forward+adjoint GCUPS on 4,096 pairs (lengths 50-600) and on 256 pairs
 (lengths 2,000-5,000, sqrt(L) checkpointing vs all cells kept).
@@ -58,6 +64,7 @@ DB_SEED, QUERY_SEED, PLANT_SEED = 2, 1002, 3002
 N_PROFILES, MEDIAN, SIGMA, MINLEN, MAXLEN = 200_000, 200, 0.668, 30, 3000
 QUERY_LEN, N_PLANT, PLANT_KMIN, PLANT_KMAX = 300, 4000, 3.0, 60.0
 MAP_CAP = 2000
+STREAM_CHUNK_POS = 12_600_000     # about 4 chunks of the C1 DB
 
 
 def log(msg):
@@ -208,6 +215,56 @@ def batch_bench(d, args):
     return out
 
 
+def stream_bench(d, q, res, nq_batch=64):
+    """The whole DB from host memory, uploaded in chunks while the previous
+    chunk is searched (cbp_search_db): the one-query search (C1) and the
+    64-query batch (C2), each timed with its upload, against the
+    resident-DB results (hit lists compared)."""
+    import numpy as np
+    from comer2_amd._ffi import SynthDB
+    from comer2_amd.host import CuBatchProcessing
+    out = {}
+    cbp = CuBatchProcessing(0, 0, 0, options=dict(max_map_pairs=MAP_CAP))
+    cbp.TransferQueries(q)
+    chunk = STREAM_CHUNK_POS
+    cbp.SearchDB(d, [0], chunk_pos=chunk)            # warm-up: buffers, query preparation
+    t = time.perf_counter()
+    r = cbp.SearchDB(d, [0], chunk_pos=chunk)[0]
+    ts = time.perf_counter() - t
+    nch = cbp.last_search_chunks()
+    same = (r["npassed"] == res["npassed"] and np.array_equal(r["alndata"].view(np.uint32), res["alndata"].view(np.uint32))
+            and np.array_equal(r["order"], res["order"]) and np.array_equal(r["aln_raw"], res["aln_raw"]))
+    cells = QUERY_LEN * d.npos
+    out["one_query"] = {"chunk_positions": chunk, "chunks": nch, "seconds": round(ts, 4),
+                        "gcups_with_upload": round(cells / ts / 1e9, 3), "identical_hits": bool(same)}
+    log("bench: streamed one-query search: %d chunks, %.3f s with the upload (%.1f GCUPS), identical %s"
+        % (nch, ts, cells / ts / 1e9, same))
+    cbp.close()
+    qb = SynthDB(3, nq_batch, 0, 0, 50, 1000, sort_desc=False)
+    cbp = CuBatchProcessing(0, 0, 0, options=dict(max_map_pairs=100))
+    cbp.TransferQueries(qb)
+    qs = list(range(nq_batch))
+    cbp.SearchDB(d, qs, chunk_pos=chunk)
+    t = time.perf_counter()
+    rs = cbp.SearchDB(d, qs, chunk_pos=chunk)
+    tb = time.perf_counter() - t
+    nchb = cbp.last_search_chunks()
+    cellsb = int(qb.lengths().sum()) * d.npos
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    rr = cbp.ProcessBatch(qs)
+    sameb = all(a["npassed"] == b["npassed"] and np.array_equal(a["alndata"].view(np.uint32), b["alndata"].view(np.uint32))
+                and np.array_equal(a["order"], b["order"]) and np.array_equal(a["aln_raw"], b["aln_raw"])
+                for a, b in zip(rs, rr))
+    out["batch64"] = {"chunk_positions": chunk, "chunks": nchb,
+                      "seconds": round(tb, 4), "gcups_with_upload": round(cellsb / tb / 1e9, 3),
+                      "identical_hits": bool(sameb)}
+    log("bench: streamed 64-query search: %.3f s with the upload (%.1f GCUPS), identical %s"
+        % (tb, cellsb / tb / 1e9, sameb))
+    cbp.close()
+    return out
+
+
 def softdp_bench(reps):
     """Soft DP (log-sum-exp forward and hand-written adjoint) at the two
     differentiation workloads: 4,096 pairs with lengths uniform in 50-600
@@ -277,6 +334,7 @@ def main():
     ap.add_argument("--no-softdp", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--softdp-reps", type=int, default=3)
+    ap.add_argument("--no-stream", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the search shards over DB profiles "
@@ -370,9 +428,12 @@ def main():
         "cpu_baseline": None,
         "softdp": None,
         "batch": None,
+        "stream": None,
     }
     if not args.no_batch:
         out["batch"] = batch_bench(d, args)
+    if not args.no_stream:
+        out["stream"] = stream_bench(d, q, res)
     if not args.no_softdp:
         out["softdp"] = softdp_bench(args.softdp_reps)
     if not args.no_cpu:

@@ -3,11 +3,11 @@
  * alignment statistics for gfx950.
  *
  * k_finalize replaces FinalizeDP/FinalizeDPPro (cudp/CuBatchDP_final.cu:61-396):
- * one thread per candidate pair walks the packed backtrack codes written by
- * the BTCK phase-1 launch from the maximum cell, collects the DIAG scores and
- * accumulates the correlation sum in the reference's order (32-score windows
- * reduced by an xor butterfly, windows added in sequence), then adds
- * sign*sqrt(|0.5 sum|).  The passed list is compacted on the host in profile
+ * one wave per candidate pair walks the packed backtrack codes written by
+ * the BTCK phase-1 launch from the maximum cell, scores the DIAG cells of
+ * each 32-cell window together and accumulates the correlation sum in the
+ * reference's order (32-score windows reduced by an xor butterfly, windows
+ * added in sequence), then adds sign*sqrt(|0.5 sum|).  The passed list is compacted on the host in profile
  * order (the reference appends under a spin lock in block-scheduling order).
  *
  * k_hist replaces InitScoreProbsDynPlmProfile + CalcScoreProbsXuninfDynPlmProfile
@@ -32,66 +32,79 @@ __device__ __forceinline__ float score_at(const ScoreCtx& c, int y, int64_t col,
     return c2_score_cell_w(c, y, col, w, &ss);
 }
 
-__global__ void k_finalize(ScoreCtx c, const int32_t* cand, int32_t ncand,
-                           const unsigned long long* keys, const uint32_t* btck,
-                           const int64_t* btck_off, FinRec* out)
+/* one wave per candidate: every lane follows the backtrace through the LDS
+ * window of the codes (P1Codes, c2_draw.h); lane j keeps the j-th DIAG cell
+ * of the current 32-cell window, the lanes score the window's cells
+ * together, and the window's correlation terms
+ *   v[t] = s[t+4] * (((s[t] + s[t+1]) + s[t+2]) + s[t+3])   (s[0..3] carried over)
+ * are summed by the same xor butterfly the reference uses
+ * (CuBatchDP_final.cu:200-292), one term per lane; windows are added in
+ * path order, so the sum has the one-thread loop's bits */
+__global__ __launch_bounds__(64) void k_finalize(ScoreCtx c, const int32_t* cand, int32_t ncand,
+                                                 const unsigned long long* keys, const uint32_t* btck,
+                                                 const int64_t* btck_off, FinRec* out)
 {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    __shared__ __align__(16) uint32_t win[C2_DRAW_WIN / 4];
+    const int i = blockIdx.x;
     if (i >= ncand) return;
+    const int lane = threadIdx.x;
     const int p = cand[i];
     const unsigned long long key = keys[p];
     const float maxsc = __uint_as_float((uint32_t)(key >> 32));
     const uint32_t lo = 0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFull);
     const int xe = (int)(lo >> 16), ye = (int)(lo & 0xFFFF);
-    const int Ldb = c.len[p], nsteps = Ldb + 63, SL = 64 * c.R;
     const int64_t col0 = c.dist[p];
-    const uint32_t* bt = btck + btck_off[i];
+    P1Codes codes;
+    codes.bt = btck + btck_off[i]; codes.nsteps = c.len[p] + 63; codes.R = c.R; codes.win = win;
     int x = xe, y = ye, psts = 0;
-    int b = 3;
     float pw[20];
     c2_pair_weights(c, p, pw);
-    float cache[36];
-    for (int k = 0; k < 36; k++) cache[k] = 0.0f;
+    float carry[4] = {0.0f, 0.0f, 0.0f, 0.0f};   /* the previous window's last four scores */
     float sumcorr = 0.0f;
     if (key != 0ull) {
+        int b = 3;
         while (b != 0) {
-            for (int k = 0; k < 4; k++) cache[k] = cache[k + 32];
-            int ndx;
-            for (ndx = 4; ndx < 36;) {
+            int nd = 0, myy = 0, myx = 0;
+            while (nd < 32) {
                 if (x < 0 || y < 0) { b = 0; break; }
-                const int s = y / SL, rr = y - s * SL, l = rr / c.R, kk = rr - l * c.R;
-                const uint32_t w = bt[((size_t)s * nsteps + (x + l)) * 64 + l];
-                b = (int)((w >> (2 * kk)) & 3u);
+                b = codes.at(y, x, lane);
                 if (b == 0) break;
                 if (b == 2) { y--; continue; }
                 if (b == 1) { x--; continue; }
-                const float sv = score_at(c, y, col0 + x, pw);
-                cache[ndx] = sv;
-                if (sv > 0.0f) psts++;
-                x--; y--; ndx++;
+                if (nd == lane) { myy = y; myx = x; }
+                x--; y--; nd++;
             }
-            float v[32];
-            for (int t = 0; t < 32; t++)
-                v[t] = (ndx <= t + 4) ? 0.0f
-                     : cache[t + 4] * (cache[t] + cache[t + 1] + cache[t + 2] + cache[t + 3]);
-            for (int off = 16; off >= 1; off >>= 1) {
-                float n[32];
-                for (int t = 0; t < 32; t++) n[t] = v[t] + v[t ^ off];
-                for (int t = 0; t < 32; t++) v[t] = n[t];
+            const float sv = (lane < nd) ? score_at(c, myy, col0 + myx, pw) : 0.0f;
+            psts += __popcll(__ballot(lane < nd && sv > 0.0f));
+            /* the four scores before this lane's: carried ones for the first lanes */
+            float prev[4];
+#pragma unroll
+            for (int k = 0; k < 4; k++) {
+                const float up = __shfl_up(sv, 4 - k);
+                const int src = lane - (4 - k);
+                const int j = lane + k;               /* carried slot when src < 0 */
+                const float cj = (j == 0) ? carry[0] : (j == 1) ? carry[1] : (j == 2) ? carry[2] : carry[3];
+                prev[k] = (src < 0) ? cj : up;
             }
-            sumcorr += v[0];
+            float v = (lane < 32 && lane < nd) ? sv * (((prev[0] + prev[1]) + prev[2]) + prev[3]) : 0.0f;
+            for (int off = 16; off >= 1; off >>= 1) v = v + __shfl_xor(v, off, 32);
+            sumcorr += __shfl(v, 0);
+#pragma unroll
+            for (int k = 0; k < 4; k++) carry[k] = __shfl(sv, 28 + k);
         }
     }
     sumcorr *= 0.5f;
     sumcorr = sumcorr < 0.0f ? -__fsqrt_rn(-sumcorr) : __fsqrt_rn(sumcorr);
-    FinRec r;
-    r.score = maxsc + sumcorr;
-    r.maxsc = maxsc;
-    r.psts = psts;
-    r.beg = ((uint32_t)(x + 1) << 16) | (uint32_t)(y + 1);
-    r.end = ((uint32_t)xe << 16) | (uint32_t)ye;
-    r.pad = 0;
-    out[i] = r;
+    if (lane == 0) {
+        FinRec r;
+        r.score = maxsc + sumcorr;
+        r.maxsc = maxsc;
+        r.psts = psts;
+        r.beg = ((uint32_t)(x + 1) << 16) | (uint32_t)(y + 1);
+        r.end = ((uint32_t)xe << 16) | (uint32_t)ye;
+        r.pad = 0;
+        out[i] = r;
+    }
 }
 
 /* FinalizeDP_ALN (cudp/CuBatchDP_aln_final.cu:61-438): alignments of the
@@ -602,7 +615,7 @@ hipError_t c2_launch_finalize(const ScoreCtx& c, const int32_t* cand, int32_t nc
                               const int64_t* btck_off, FinRec* out, hipStream_t stream)
 {
     if (ncand <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_finalize, dim3((ncand + 63) / 64), dim3(64), 0, stream, c, cand, ncand, keys, btck, btck_off, out);
+    hipLaunchKernelGGL(k_finalize, dim3(ncand), dim3(64), 0, stream, c, cand, ncand, keys, btck, btck_off, out);
     return hipGetLastError();
 }
 
