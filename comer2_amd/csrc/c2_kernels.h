@@ -45,6 +45,7 @@ struct P1QueueArgs {
     unsigned* head;             /* pull counter, zero before the launch */
     float* botw;                /* per-wave stripe hand-off rows [grid*NW][5][bot_ld] (multi-stripe bins) */
     int64_t bot_ld;             /* >= the columns of any segment in the queue */
+    const int32_t* ntiles_dev = nullptr;   /* tile count written on the device (replaces ntiles) */
 };
 
 /* everything needed to recompute one score-matrix cell */
@@ -100,6 +101,12 @@ struct MapArgs {
      * pairs are cut into column chunks */
     const int4* sitems;
     int32_t nsitems;
+    /* pairs of the launch: npairs (host count), or with dcnt the device
+     * counters of the query's lists (DL_*), group 0 the long-pair group,
+     * group 1 the rest, -1 every drawn pair */
+    int32_t npairs = 0;
+    const int32_t* dcnt = nullptr;
+    int32_t group = -1;
 };
 /* columns per item of the dynamic-score pass */
 constexpr int C2_MAP_SCORE_COLS = 256;
@@ -114,8 +121,63 @@ struct AlnArgs {
     char* alns; char* alntmp;        /* [n][3][alnstride] */
     int64_t alnstride;
     int32_t* res;                    /* [n][8] as MapArgs::res (slots 0..5) */
+    const int32_t* dcnt = nullptr;   /* device counters: n = dcnt[DL_NSEL] */
 };
 hipError_t c2_launch_aln_final(const ScoreCtx& c, const AlnArgs& a, int n, hipStream_t stream);
+struct StatIn {
+    int32_t Ldb; float deno; float score; float psts;
+};
+
+/* ---- stage lists built on the device (c2_dlist.hip) ----
+ * per query: a block of counters (DL_*), the candidate, pass and drawing
+ * lists with fixed capacities, and the MAP plan; a list beyond its
+ * capacity sets a DL_OVF bit and the query falls back to host-built lists */
+enum {
+    DL_NC = 0,          /* candidates (phase-1 maximum above the score threshold) */
+    DL_NPASS,           /* passed pairs */
+    DL_NSEL,            /* pairs drawn (MAP or phase-1 drawing) */
+    DL_NA,              /* long-pair group of the MAP passes (one-query calls) */
+    DL_NSITEMS,         /* MAP score items */
+    DL_ITA,             /* score items of the long-pair group */
+    DL_MAXLEN,          /* longest passed profile */
+    DL_MAXLD,           /* longest drawn profile */
+    DL_OVF,             /* overflow bits */
+    DL_PAD_,
+    DL_NPOSITS = 10,    /* int64: positions of the passed profiles */
+    DL_CELLS = 12,      /* int64: MAP cells */
+    DL_NCNT = 16
+};
+enum { DL_OVF_CAND = 1, DL_OVF_BTCK = 2, DL_OVF_PASS = 4, DL_OVF_SEL = 8, DL_OVF_MAP = 16 };
+constexpr int DL_TILE = 1024;       /* profiles per compaction tile */
+constexpr int DL_SORT = 16384;      /* largest list sorted in LDS (passed pairs) */
+struct DlBufs {
+    int32_t* cnt;                                   /* [DL_NCNT] */
+    int32_t* tilecnt; int32_t* tileoff;             /* [nprof / DL_TILE] */
+    int32_t* cand; int32_t* pe; int64_t* off;       /* [capC], [capC], [capC + 1] */
+    int32_t capC; int64_t capW;                     /* candidates; backtrack-code words */
+    int32_t* pass_pro; int32_t* passfin; StatIn* statin;   /* [capP] */
+    float* aln; float* epa; int32_t* order;         /* [capP][CBP_NFIELDS], [capP], [capP] */
+    int32_t capP;
+    int32_t* sel; MapPair* mpairs; int32_t* morder; /* [capM] */
+    int4* sitems; int32_t capM; int64_t capItems;
+    int64_t capCells, capHand;                      /* MAP matrices (cells), hand-off floats */
+    int32_t* apro; int32_t* acand;                  /* [capM] MAPALN=0 lists */
+    int32_t* res; float* p1save;                    /* [capM][8], [capM][3] */
+};
+struct DlSelArgs {
+    const int32_t* len;                             /* chunk profile lengths */
+    float logevthld;
+    int32_t cap, mapaln, mR, mns, modinuse, split;
+};
+hipError_t c2_dl_candidates(const unsigned long long* keys, int32_t np, float thld, const DlBufs& b,
+                            const int32_t* len, int32_t nstripes, P1Args* dargs, P1Tile* tiles, int32_t qslot,
+                            int32_t* ntiles, hipStream_t st);
+hipError_t c2_dl_pass(const FinRec* fin, const int32_t* cand, const int32_t* len, const float* eno, float thld,
+                      const DlBufs& b, hipStream_t st);
+hipError_t c2_dl_order_select(const double* statout, const DlBufs& b, const DlSelArgs& sa, hipStream_t st);
+hipError_t c2_dl_aln_lists(const DlBufs& b, hipStream_t st);
+hipError_t c2_dl_results(const DlBufs& b, hipStream_t st);
+
 /* MAP geometry for a query: rows per lane and waves per pair */
 void c2_map_geometry(int Lq, int* R, int* ns, int* nw);
 /* score pass, then the forward pass and log Z on stream while the backward
@@ -133,14 +195,15 @@ hipError_t c2_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t stre
 /* score weights of the query against every chunk profile into wts [nprof][20] */
 hipError_t c2_launch_weights(const float* qbkg, const float* bkg, int nprof, float* wts, hipStream_t stream);
 hipError_t c2_launch_zero_u32(uint32_t* p, int64_t n, hipStream_t stream);
+/* ncand_dev (or npro_dev, n_dev below): a device count replacing the host
+ * count, which then only sizes the grid */
 hipError_t c2_launch_finalize(const ScoreCtx& c, const int32_t* cand, int32_t ncand,
                               const unsigned long long* keys, const uint32_t* btck,
-                              const int64_t* btck_off, FinRec* out, hipStream_t stream);
+                              const int64_t* btck_off, FinRec* out, hipStream_t stream,
+                              const int32_t* ncand_dev = nullptr);
 hipError_t c2_launch_hist(const ScoreCtx& c, const int32_t* pro, int32_t npro, int xuninf,
-                          int32_t* counts /* [npro][641] */, hipStream_t stream);
-struct StatIn {
-    int32_t Ldb; float deno; float score; float psts;
-};
+                          int32_t* counts /* [npro][641] */, hipStream_t stream,
+                          const int32_t* npro_dev = nullptr);
 struct StatParams {
     int32_t Lq; float qeno; float searchspace; int32_t ssemodel;
     float ung_lambda, ung_K, gap_lambda, gap_K;
@@ -148,7 +211,7 @@ struct StatParams {
 /* crow: the counts row of pair i (nullptr: row i) */
 hipError_t c2_launch_stats(const StatParams& sp, const float* nnw /*device [93]*/,
                            const int32_t* counts, const int32_t* crow, const StatIn* in, int32_t n,
-                           double* out /* [n][10] */, hipStream_t stream);
+                           double* out /* [n][10] */, hipStream_t stream, const int32_t* n_dev = nullptr);
 hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream_t stream);
 /* persistent workgroups that fill the device once for k_phase1q<R,NW,btck> with lds bytes */
 int c2_phase1q_grid(int R, int NW, bool btck, size_t lds);

@@ -243,6 +243,9 @@ int c2_build_segments(cbp_handle* h, const cbp_profiles* s)
     if (h->seg_p.back() != s->nprof) h->seg_p.push_back(s->nprof);
     h->nseg = (int32_t)h->seg_p.size() - 1;
     h->seg_maxcols = 0;
+    h->chunk_maxlen = 0;
+    for (int32_t p = 0; p < s->nprof; p++)
+        if (s->len[p] > h->chunk_maxlen) h->chunk_maxlen = s->len[p];
     for (int32_t i = 0; i < h->nseg; i++) {
         const int32_t pl = h->seg_p[i + 1] - 1;
         const int64_t cols = (int64_t)s->dist[pl] + s->len[pl] - (int64_t)s->dist[h->seg_p[i]];

@@ -143,6 +143,29 @@ struct StreamOut {
     std::vector<char> strs;
 };
 
+/* device-built stage lists of one query (c2_dpipe.cpp): the lists and
+ * records that outlive the query's chain, and its counters in pinned host
+ * memory */
+struct DlQuery {
+    DBuf cnt, aln, epa, order, sel, res, p1save, malns;
+    DBuf dargs, tiles, ntiles;          /* the backtrack pass's queue: arguments, tiles, tile count */
+    int32_t* hcnt = nullptr;            /* pinned copy of cnt */
+    ~DlQuery() { if (hcnt) (void)hipHostFree(hcnt); }
+};
+/* per stream slot: the pools a query's chain uses while it runs (the
+ * slot's queries run one after another on its stream) */
+struct DlSlot {
+    DBuf tilecnt, tileoff, cand, pe, off, btck, botw, fin, pass_pro, passfin, statin, counts, statout;
+    DBuf mpairs, morder, sitems, apro, acand, mF, mB, mSd, mbt, mhand, mstat, malntmp;
+};
+/* a query's results from the device lists, as the caller sees them */
+struct DlResult {
+    std::vector<float> aln, epa;
+    std::vector<int32_t> order, map;
+    std::vector<float> p1save;
+    std::vector<char> strs;
+};
+
 constexpr int C2_NSTREAMS = 4;
 constexpr size_t C2_QCACHE_MAX = 128;
 /* one-query calls run the candidates' histograms ahead of the finalize when
@@ -219,6 +242,17 @@ struct cbp_handle {
      * calls: the long-pair group and the rest) */
     hipEvent_t mfork[2] = {nullptr, nullptr}, mjoin[2] = {nullptr, nullptr};
     uint64_t chunk_gen = 0;             /* bumped by every chunk upload / tuning change */
+    /* device-built stage lists (default on): per query and per stream slot
+     * buffers, capacities, results; the host-decided stages remain the
+     * fallback of a query that exceeds a capacity */
+    bool use_dlists = true;
+    std::vector<std::unique_ptr<DlQuery>> dlq;
+    DlSlot dls[C2_NSTREAMS];
+    std::vector<DlResult> dlres;
+    int32_t dl_capC = 65536, dl_capP = 16384;
+    int64_t dl_capW = (int64_t)1 << 27, dl_capCells = (int64_t)1 << 28;
+    int32_t chunk_maxlen = 0;           /* longest profile of the chunk */
+    int32_t last_dl_fallbacks = 0;      /* queries of the last call that fell back */
     /* whole-DB search in chunks: two device chunk sets, pinned staging
      * buffers and the copy stream of the copy thread, merged results */
     DevSet dsets[2];
@@ -265,5 +299,15 @@ int c2_build_segments(cbp_handle* h, const cbp_profiles* s);
 int c2_run_phase1(cbp_handle* h, int qidx);
 int c2_run_queries(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* scorethld,
                    const float* logevthld, cbp_results* outs);
+/* the same sequence with the stage lists built on the device */
+int c2_run_queries_dev(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* scorethld,
+                       const float* logevthld, cbp_results* outs);
+/* stage helpers shared by both sequences (c2_pipe.cpp) */
+hipStream_t c2_slot_stream(cbp_handle* h, int slot);
+int c2_ensure_fork_events(cbp_handle* h);
+int c2_phase1_prepare(cbp_handle* h, QWork& w);
+int c2_phase1_launch(cbp_handle* h, QWork& w, bool timed, bool copy);
+int c2_queue_scan_prepare(cbp_handle* h, int nq);
+int c2_queue_scan_launch(cbp_handle* h, int nq, bool timed, bool copy);
 
 #endif

@@ -438,12 +438,13 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1q(P1QueueArgs qa)
     __shared__ int tile_s;
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
     float* botw = qa.botw ? qa.botw + ((size_t)blockIdx.x * NW + wv) * 5 * qa.bot_ld : nullptr;
+    const int ntiles = qa.ntiles_dev ? __builtin_amdgcn_readfirstlane(*qa.ntiles_dev) : qa.ntiles;
     for (;;) {
         if (threadIdx.x == 0) tile_s = (int)atomicAdd(qa.head, 1u);
         __syncthreads();
         const int ti = __builtin_amdgcn_readfirstlane(tile_s);   /* uniform: the tile's arguments load as scalars */
         __syncthreads();
-        if (ti >= qa.ntiles) break;
+        if (ti >= ntiles) break;
         const int tq = __builtin_amdgcn_readfirstlane(qa.tiles[ti].q);
         const int tg = __builtin_amdgcn_readfirstlane(qa.tiles[ti].g);
         /* the arguments are read-only for the whole launch: read through the

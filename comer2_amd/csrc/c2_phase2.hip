@@ -32,21 +32,9 @@ __device__ __forceinline__ float score_at(const ScoreCtx& c, int y, int64_t col,
     return c2_score_cell_w(c, y, col, w, &ss);
 }
 
-/* one wave per candidate: every lane follows the backtrace through the LDS
- * window of the codes (P1Codes, c2_draw.h); lane j keeps the j-th DIAG cell
- * of the current 32-cell window, the lanes score the window's cells
- * together, and the window's correlation terms
- *   v[t] = s[t+4] * (((s[t] + s[t+1]) + s[t+2]) + s[t+3])   (s[0..3] carried over)
- * are summed by the same xor butterfly the reference uses
- * (CuBatchDP_final.cu:200-292), one term per lane; windows are added in
- * path order, so the sum has the one-thread loop's bits */
-__global__ __launch_bounds__(64) void k_finalize(ScoreCtx c, const int32_t* cand, int32_t ncand,
-                                                 const unsigned long long* keys, const uint32_t* btck,
-                                                 const int64_t* btck_off, FinRec* out)
+__device__ void finalize_one(const ScoreCtx& c, const int32_t* cand, const unsigned long long* keys,
+                             const uint32_t* btck, const int64_t* btck_off, FinRec* out, uint32_t* win, int i)
 {
-    __shared__ __align__(16) uint32_t win[C2_DRAW_WIN / 4];
-    const int i = blockIdx.x;
-    if (i >= ncand) return;
     const int lane = threadIdx.x;
     const int p = cand[i];
     const unsigned long long key = keys[p];
@@ -107,14 +95,31 @@ __global__ __launch_bounds__(64) void k_finalize(ScoreCtx c, const int32_t* cand
     }
 }
 
+/* one wave per candidate: every lane follows the backtrace through the LDS
+ * window of the codes (P1Codes, c2_draw.h); lane j keeps the j-th DIAG cell
+ * of the current 32-cell window, the lanes score the window's cells
+ * together, and the window's correlation terms
+ *   v[t] = s[t+4] * (((s[t] + s[t+1]) + s[t+2]) + s[t+3])   (s[0..3] carried over)
+ * are summed by the same xor butterfly the reference uses
+ * (CuBatchDP_final.cu:200-292), one term per lane; windows are added in
+ * path order, so the sum has the one-thread loop's bits */
+__global__ __launch_bounds__(64) void k_finalize(ScoreCtx c, const int32_t* cand, int32_t ncand,
+                                                 const unsigned long long* keys, const uint32_t* btck,
+                                                 const int64_t* btck_off, FinRec* out, const int32_t* ncand_dev)
+{
+    __shared__ __align__(16) uint32_t win[C2_DRAW_WIN / 4];
+    const int n = ncand_dev ? __builtin_amdgcn_readfirstlane(*ncand_dev) : ncand;
+    for (int i = blockIdx.x; i < n; i += gridDim.x)
+        finalize_one(c, cand, keys, btck, btck_off, out, win, i);
+}
 /* FinalizeDP_ALN (cudp/CuBatchDP_aln_final.cu:61-438): alignments of the
  * MAPALN=0 path drawn from the phase-1 backtrace, one wave per pair;
  * end coordinates are the phase-1 maximum's */
 __global__ __launch_bounds__(64) void k_aln_final(ScoreCtx c, AlnArgs a, int n)
 {
     __shared__ __align__(16) uint32_t win[C2_DRAW_WIN / 4];
-    const int i = blockIdx.x;
-    if (i >= n) return;
+    const int nn = a.dcnt ? __builtin_amdgcn_readfirstlane(a.dcnt[DL_NSEL]) : n;
+    for (int i = blockIdx.x; i < nn; i += gridDim.x) {
     const int p = a.pro[i];
     const unsigned long long key = a.keys[p];
     const uint32_t lo = 0xFFFFFFFFu - (uint32_t)(key & 0xFFFFFFFFull);
@@ -126,6 +131,7 @@ __global__ __launch_bounds__(64) void k_aln_final(ScoreCtx c, AlnArgs a, int n)
     c2_walk_draw(c, p, xe, ye, codes, a.alns + (size_t)i * 3 * stride, a.alntmp + (size_t)i * 3 * stride,
                  stride, o);
     if (threadIdx.x == 0) o[5] = (int32_t)(((uint32_t)xe << 16) | (uint32_t)ye);
+    }
 }
 
 constexpr int NSC = 641;
@@ -135,16 +141,20 @@ constexpr int NSC = 641;
 constexpr int HTY = 128;
 constexpr int HQW = 21;
 
-__global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, int xuninf, int32_t* counts)
+__global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, int xuninf, int32_t* counts,
+                                              int32_t npro, const int32_t* npro_dev)
 {
     constexpr int NWH = 4;                  /* waves of the 256-thread workgroup, one histogram each */
     __shared__ int h[NWH][NSC];
     __shared__ float sq[HTY * HQW];
     __shared__ float sqt[30 * HTY];
     __shared__ unsigned char sx[HTY];
+    const int npairs = npro_dev ? __builtin_amdgcn_readfirstlane(*npro_dev) : npro;
+    for (int pi = blockIdx.x; pi < npairs; pi += gridDim.x) {
+    __syncthreads();
     for (int i = threadIdx.x; i < NWH * NSC; i += blockDim.x) (&h[0][0])[i] = 0;
     int* hw = h[threadIdx.x >> 6];
-    const int p = pro[blockIdx.x];
+    const int p = pro[pi];
     const int Ldb = c.len[p];
     const int64_t col0 = c.dist[p];
     float w[20];
@@ -206,7 +216,8 @@ __global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, in
     }
     __syncthreads();
     for (int i = threadIdx.x; i < NSC; i += blockDim.x)
-        counts[(size_t)blockIdx.x * NSC + i] = h[0][i] + h[1][i] + h[2][i] + h[3][i];
+        counts[(size_t)pi * NSC + i] = h[0][i] + h[1][i] + h[2][i] + h[3][i];
+    }
 }
 
 /* ---------------- FP64 statistics (one wave per pair) ---------------- */
@@ -442,16 +453,10 @@ __device__ double prob_len_jw(double* C, double* NW, const double* pu, const int
 /* Karlin-Altschul statistics and E-values of one passed pair per wave
  * (CalcStatParamsDynPlmProfile + CalcSignificanceSS18/ObsDynPlmProfile,
  * cuss/CuBatchSS_dynplm.cu:193-674), FP64 */
-__global__ __launch_bounds__(256) void k_stats(StatParams sp, const float* nnw, const int32_t* counts,
-                                               const int32_t* crow, const StatIn* in, int32_t n, double* out)
+__device__ void stats_one(const StatParams& sp, const float* nnw, const int32_t* counts, const int32_t* crow,
+                          const StatIn* in, double* out, int i, double* P, double* NW, double* pu, int* L, int* Rr)
 {
-    __shared__ double sP[4][NUNS * SCL], sNW[4][NUNS * SCL], spu[4][NUNS];
-    __shared__ int sL[4][NUNS], sR[4][NUNS];
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + wv;
-    if (i >= n) return;
-    double* P = sP[wv]; double* NW = sNW[wv]; double* pu = spu[wv];
-    int* L = sL[wv]; int* Rr = sR[wv];
+    const int lane = threadIdx.x & 63;
     const int32_t* cnt = counts + (size_t)(crow ? crow[i] : i) * NSC;
     if (lane < NUNS) {
         int l, r;
@@ -595,6 +600,18 @@ __global__ __launch_bounds__(256) void k_stats(StatParams sp, const float* nnw, 
     o[8] = logexp + log(ss / l1l2);
 }
 
+__global__ __launch_bounds__(256) void k_stats(StatParams sp, const float* nnw, const int32_t* counts,
+                                               const int32_t* crow, const StatIn* in, int32_t n, double* out,
+                                               const int32_t* n_dev)
+{
+    __shared__ double sP[4][NUNS * SCL], sNW[4][NUNS * SCL], spu[4][NUNS];
+    __shared__ int sL[4][NUNS], sR[4][NUNS];
+    const int wv = threadIdx.x >> 6;
+    const int nn = n_dev ? __builtin_amdgcn_readfirstlane(*n_dev) : n;
+    for (int i = blockIdx.x * 4 + wv; i < nn; i += gridDim.x * 4)
+        stats_one(sp, nnw, counts, crow, in, out, i, sP[wv], sNW[wv], spu[wv], sL[wv], sR[wv]);
+}
+
 /* S and its SS component for one pair, row-major [Lq][Ldb] */
 __global__ __launch_bounds__(256) void k_score_block(ScoreCtx c, int p, float* S, float* mod)
 {
@@ -610,12 +627,17 @@ __global__ __launch_bounds__(256) void k_score_block(ScoreCtx c, int p, float* S
 
 }  // namespace
 
+/* a device count: the host count is a capacity; the grid loops */
+static int grid_of(int n, const void* dev, int cap) { return dev && n > cap ? cap : n; }
+
 hipError_t c2_launch_finalize(const ScoreCtx& c, const int32_t* cand, int32_t ncand,
                               const unsigned long long* keys, const uint32_t* btck,
-                              const int64_t* btck_off, FinRec* out, hipStream_t stream)
+                              const int64_t* btck_off, FinRec* out, hipStream_t stream,
+                              const int32_t* ncand_dev)
 {
     if (ncand <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_finalize, dim3(ncand), dim3(64), 0, stream, c, cand, ncand, keys, btck, btck_off, out);
+    hipLaunchKernelGGL(k_finalize, dim3(grid_of(ncand, ncand_dev, 8192)), dim3(64), 0, stream, c, cand, ncand,
+                       keys, btck, btck_off, out, ncand_dev);
     return hipGetLastError();
 }
 
@@ -624,23 +646,25 @@ hipError_t c2_launch_aln_final(const ScoreCtx& c, const AlnArgs& a, int n, hipSt
     if (n <= 0) return hipSuccess;
     if (!a.pro || !a.cand || !a.keys || !a.btck || !a.btck_off || !a.alns || !a.alntmp || !a.res)
         return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_aln_final, dim3(n), dim3(64), 0, stream, c, a, n);
+    hipLaunchKernelGGL(k_aln_final, dim3(grid_of(n, a.dcnt, 8192)), dim3(64), 0, stream, c, a, n);
     return hipGetLastError();
 }
 
 hipError_t c2_launch_hist(const ScoreCtx& c, const int32_t* pro, int32_t npro, int xuninf,
-                          int32_t* counts, hipStream_t stream)
+                          int32_t* counts, hipStream_t stream, const int32_t* npro_dev)
 {
     if (npro <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_hist, dim3(npro), dim3(256), 0, stream, c, pro, xuninf, counts);
+    hipLaunchKernelGGL(k_hist, dim3(grid_of(npro, npro_dev, 4096)), dim3(256), 0, stream, c, pro, xuninf, counts,
+                       npro, npro_dev);
     return hipGetLastError();
 }
 
 hipError_t c2_launch_stats(const StatParams& sp, const float* nnw, const int32_t* counts, const int32_t* crow,
-                           const StatIn* in, int32_t n, double* out, hipStream_t stream)
+                           const StatIn* in, int32_t n, double* out, hipStream_t stream, const int32_t* n_dev)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stats, dim3((n + 3) / 4), dim3(256), 0, stream, sp, nnw, counts, crow, in, n, out);
+    hipLaunchKernelGGL(k_stats, dim3(grid_of((n + 3) / 4, n_dev, 2048)), dim3(256), 0, stream, sp, nnw, counts,
+                       crow, in, n, out, n_dev);
     return hipGetLastError();
 }
 

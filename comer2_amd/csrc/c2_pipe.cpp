@@ -65,7 +65,7 @@ int phase1_prepare(cbp_handle* h, QWork& w)
 
 /* 1b. phase-1 maxima of every DB profile for the query (kernels and one copy;
  * capturable) */
-int phase1_launch(cbp_handle* h, QWork& w, bool timed)
+int phase1_launch(cbp_handle* h, QWork& w, bool timed, bool copy = true)
 {
     DevQuery* dq = w.dq;
     const int np = h->dchunk.nprof;
@@ -82,7 +82,7 @@ int phase1_launch(cbp_handle* h, QWork& w, bool timed)
     if (timed) PCHK(hipEventRecord(w.e0, w.st));
     PCHK(c2_launch_phase1(a, dq->hp.R, nw, false, w.st));
     if (timed) PCHK(hipEventRecord(w.e1, w.st));
-    PCHK(hipMemcpyAsync(w.hk, w.dk, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, w.st));
+    if (copy) PCHK(hipMemcpyAsync(w.hk, w.dk, sizeof(unsigned long long) * np, hipMemcpyDeviceToHost, w.st));
     return CBP_OK;
 }
 
@@ -236,7 +236,7 @@ int queue_scan_prepare(cbp_handle* h, int nq)
 
 /* 1b (batch). maxima of every (query, DB profile) pair of the batch
  * (capturable) */
-int queue_scan_launch(cbp_handle* h, int nq, bool timed)
+int queue_scan_launch(cbp_handle* h, int nq, bool timed, bool copy = true)
 {
     const int np = h->dchunk.nprof;
     int rc;
@@ -249,8 +249,9 @@ int queue_scan_launch(cbp_handle* h, int nq, bool timed)
     if (timed) PCHK(hipEventRecord(h->ev0, h->stream));
     if ((rc = queue_launch(h, h->qscan, false))) return rc;
     if (timed) PCHK(hipEventRecord(h->ev1, h->stream));
-    PCHK(hipMemcpyAsync(h->hqkeys, h->qkeys.p, sizeof(unsigned long long) * (size_t)nq * np,
-                        hipMemcpyDeviceToHost, h->stream));
+    if (copy)
+        PCHK(hipMemcpyAsync(h->hqkeys, h->qkeys.p, sizeof(unsigned long long) * (size_t)nq * np,
+                            hipMemcpyDeviceToHost, h->stream));
     return CBP_OK;
 }
 
@@ -736,6 +737,14 @@ void drop_graphs(cbp_handle* h) { c2_drop_graphs(h); }
 
 }  // namespace
 
+/* the stage helpers the device-list sequence (c2_dpipe.cpp) shares */
+hipStream_t c2_slot_stream(cbp_handle* h, int slot) { return slot_stream(h, slot); }
+int c2_ensure_fork_events(cbp_handle* h) { return ensure_fork_events(h); }
+int c2_phase1_prepare(cbp_handle* h, QWork& w) { return phase1_prepare(h, w); }
+int c2_phase1_launch(cbp_handle* h, QWork& w, bool timed, bool copy) { return phase1_launch(h, w, timed, copy); }
+int c2_queue_scan_prepare(cbp_handle* h, int nq) { return queue_scan_prepare(h, nq); }
+int c2_queue_scan_launch(cbp_handle* h, int nq, bool timed, bool copy) { return queue_scan_launch(h, nq, timed, copy); }
+
 void c2_drop_graphs(cbp_handle* h)
 {
     for (auto& g : h->graphs) if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
@@ -854,6 +863,7 @@ static int run_queries_guarded(cbp_handle* h, const int32_t* qidx, int32_t nq, c
                                const float* logevthld, cbp_results* outs)
 {
     try {
+        if (h->use_dlists) return c2_run_queries_dev(h, qidx, nq, scorethld, logevthld, outs);
         return c2_run_queries(h, qidx, nq, scorethld, logevthld, outs);
     } catch (const std::bad_alloc&) {
         return c2_fail(h, CBP_ERR_MEM, "host memory allocation failed");
@@ -887,6 +897,30 @@ extern "C" int cbp_set_graph(cbp_handle* h, int32_t on)
     if (!h) return CBP_ERR_ARG;
     h->use_graph = on != 0;
     drop_graphs(h);
+    return CBP_OK;
+}
+
+extern "C" int cbp_set_device_lists(cbp_handle* h, int32_t on)
+{
+    if (!h) return CBP_ERR_ARG;
+    h->use_dlists = on != 0;
+    drop_graphs(h);
+    return CBP_OK;
+}
+
+extern "C" int cbp_set_device_list_caps(cbp_handle* h, int32_t cand, int32_t passed, int64_t code_words,
+                                        int64_t map_cells)
+{
+    if (!h || cand < 1 || passed < 1 || passed > DL_SORT || code_words < 64 || map_cells < 64) return CBP_ERR_ARG;
+    h->dl_capC = cand; h->dl_capP = passed; h->dl_capW = code_words; h->dl_capCells = map_cells;
+    drop_graphs(h);
+    return CBP_OK;
+}
+
+extern "C" int cbp_last_fallbacks(const cbp_handle* h, int32_t* n)
+{
+    if (!h || !n) return CBP_ERR_ARG;
+    *n = h->last_dl_fallbacks;
     return CBP_OK;
 }
 

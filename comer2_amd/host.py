@@ -52,6 +52,9 @@ def lib():
     L.cbp_set_tuning.argtypes = [H, C.c_int32, C.c_int32, C.c_int32]
     L.cbp_set_graph.argtypes = [H, C.c_int32]
     L.cbp_set_queue.argtypes = [H, C.c_int32]
+    L.cbp_set_device_lists.argtypes = [H, C.c_int32]
+    L.cbp_set_device_list_caps.argtypes = [H, C.c_int32, C.c_int32, C.c_int64, C.c_int64]
+    L.cbp_last_fallbacks.argtypes = [H, C.POINTER(C.c_int32)]
     L.cbp_chunk_limits.argtypes = [H, C.c_uint64] + [C.POINTER(C.c_uint64)] * 4
     L.cbp_autotune.argtypes = [H, C.c_int32, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     L.cbp_process.argtypes = [H, C.c_int32, C.c_float, C.c_float, C.POINTER(Results)]
@@ -196,6 +199,18 @@ class CuBatchProcessing:
 
     def set_tuning(self, seg_cols=4096, rmax=5, nw=8):
         self._check(self.L.cbp_set_tuning(self.h, seg_cols, rmax, nw))
+
+    def set_device_lists(self, on):
+        """stage lists built on the device (default) or on the host"""
+        self._check(self.L.cbp_set_device_lists(self.h, 1 if on else 0))
+
+    def set_device_list_caps(self, cand, passed, code_words, map_cells):
+        self._check(self.L.cbp_set_device_list_caps(self.h, cand, passed, code_words, map_cells))
+
+    def last_fallbacks(self):
+        n = C.c_int32()
+        self._check(self.L.cbp_last_fallbacks(self.h, C.byref(n)))
+        return n.value
 
     def chunk_limits(self, qlen):
         """(max DB positions, max profiles, pass-2 positions, pass-2 profiles)

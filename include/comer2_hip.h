@@ -264,6 +264,26 @@ int  cbp_set_graph(cbp_handle* h, int32_t on);
  * cualn/DevCommThread.cpp:296-356) */
 int  cbp_set_queue(cbp_handle* h, int32_t on);
 
+/* stage lists built on the device (on = 1, the default): the candidate,
+ * pass and drawing lists, the (log E, profile) order and the MAP plan of
+ * cbp_process / cbp_process_batch are ordered compactions and sorts on the
+ * device, so no host decision separates the stages and cbp_set_graph
+ * captures the whole sequence (phase 1 to the records); a call in which a
+ * list exceeds its fixed capacity runs again with host-built lists (on = 0
+ * always does).  Results are identical either way.  The reference builds
+ * its pass list on the device too (cudp/CuBatchDP_final.cu:250-264) and
+ * schedules statistics and MAP from the device
+ * (cuss/CuBatchSS_dynplm.cu:697-830, cumapdp/CuBatchMAPDP_prnt.cu:18-115) */
+int  cbp_set_device_lists(cbp_handle* h, int32_t on);
+/* capacities of the device lists per query: candidates, passed pairs (at
+ * most 16384, the order is sorted in LDS), backtrack-code words and MAP
+ * cells per stream slot (defaults 65536, 16384, 2^27, 2^28) */
+int  cbp_set_device_list_caps(cbp_handle* h, int32_t cand, int32_t passed, int64_t code_words,
+                              int64_t map_cells);
+/* queries of the last cbp_process* call that ran again with host-built
+ * lists because a device list exceeded its capacity */
+int  cbp_last_fallbacks(const cbp_handle* h, int32_t* n);
+
 /* device-side elapsed time of the last cbp_phase1_keys / cbp_process call's
  * dominant kernel (ms, HIP events on its stream) and the number of launches */
 int  cbp_last_kernel_ms(const cbp_handle* h, double* ms, int32_t* nlaunch);

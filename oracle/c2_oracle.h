@@ -126,6 +126,8 @@ void or_set_flags(int flags);
 /* c2_logf / c2_expf of c2_fmath.h over an array */
 void or_vlogf(const float* x, float* y, int64_t n);
 void or_vexpf(const float* x, float* y, int64_t n);
+/* c2_lse1p (log(1 + e^d), the MAP passes' two-term log-sum-exp) over an array */
+void or_vlse1p(const float* x, float* y, int64_t n);
 #ifdef __cplusplus
 }
 #endif

@@ -249,6 +249,28 @@ def test_expf_within_stated_ulp_and_range():
     assert rel.max() < 1e-6
 
 
+def test_lse1p_table_within_stated_error():
+    """c2_lse1p, the MAP passes' log(1 + e^d): 1.2e-7 absolute over (-16, 0]
+    against log1p(exp(d)) in double, interval edges and the clamped top
+    interval included; exactly 0 at and below the cut-off -16; never
+    decreasing by more than rounding over a dense grid."""
+    rng = np.random.default_rng(5)
+    edges = np.float32(-16.0) + np.float32(0.25) * np.arange(1, 65, dtype=np.float32)
+    x = np.concatenate([rng.uniform(-16.0, 0.0, 2_000_000).astype(np.float32),
+                        edges, np.nextafter(edges, np.float32(-20.0)),
+                        np.array([0.0, -1e-8, -15.999999, np.nextafter(np.float32(-16.0), np.float32(0.0))],
+                                 np.float32)])
+    x = x[x > np.float32(-16.0)]
+    y = _vf("or_vlse1p", x)
+    err = np.abs(y.astype(np.float64) - np.log1p(np.exp(x.astype(np.float64))))
+    assert err.max() <= 1.2e-7, float(err.max())
+    below = np.array([-16.0, -16.5, -99.0, -1e30], np.float32)
+    assert np.all(_vf("or_vlse1p", below) == 0.0)
+    g = np.linspace(-16.0, 0.0, 400_001).astype(np.float32)[1:]
+    yg = _vf("or_vlse1p", g).astype(np.float64)
+    assert np.diff(yg).min() >= -1.2e-7
+
+
 # ---------------------------------------------------------------------------
 # SS18 E-value invariants
 
