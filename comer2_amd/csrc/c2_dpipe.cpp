@@ -79,7 +79,8 @@ int dl_alloc_query(cbp_handle* h, DlQuery& q, const DlCaps& k)
     return CBP_OK;
 }
 
-int dl_alloc_slot(cbp_handle* h, DlSlot& s, const DlCaps& k, int64_t botw_floats)
+/* ahead: histogram rows for every candidate (one-query calls) */
+int dl_alloc_slot(cbp_handle* h, DlSlot& s, const DlCaps& k, int64_t botw_floats, bool ahead)
 {
     const size_t np = (size_t)h->dchunk.nprof, ntile = (np + DL_TILE - 1) / DL_TILE;
     int rc;
@@ -94,7 +95,7 @@ int dl_alloc_slot(cbp_handle* h, DlSlot& s, const DlCaps& k, int64_t botw_floats
     if ((rc = c2_dalloc(h, s.pass_pro, sizeof(int32_t) * (size_t)k.capP))) return rc;
     if ((rc = c2_dalloc(h, s.passfin, sizeof(int32_t) * (size_t)k.capP))) return rc;
     if ((rc = c2_dalloc(h, s.statin, sizeof(StatIn) * (size_t)k.capP))) return rc;
-    if ((rc = c2_dalloc(h, s.counts, sizeof(int32_t) * 641 * (size_t)k.capP))) return rc;
+    if ((rc = c2_dalloc(h, s.counts, sizeof(int32_t) * 641 * (size_t)(ahead ? k.capC : k.capP)))) return rc;
     if ((rc = c2_dalloc(h, s.statout, sizeof(double) * 10 * (size_t)k.capP))) return rc;
     if ((rc = c2_dalloc(h, s.mpairs, sizeof(MapPair) * (size_t)k.capM))) return rc;
     if ((rc = c2_dalloc(h, s.morder, sizeof(int32_t) * (size_t)k.capM))) return rc;
@@ -165,16 +166,32 @@ int dl_chain(cbp_handle* h, int i, const DlPlan& P)
     qa.head = (unsigned*)(q.cnt.as<int32_t>() + DL_PAD_);
     qa.botw = hp.nstripes > 1 ? sl.botw.as<float>() : nullptr;
     qa.bot_ld = h->chunk_maxlen;
-    DCHK(c2_launch_phase1q(qa, hp.R, 2, true, P.grid[i], P.lds[i], st));
     ScoreCtx sc;
     c2_fill_scorectx(h, w.dq, &sc);
+    /* a query on its own: the candidates' histograms (they need only the
+     * pairs) run on a second stream beside the backtrack pass and finalize,
+     * and the statistics read the passed pairs' rows through passfin */
+    const bool ahead = P.split;
+    if (ahead) {
+        DCHK(hipEventRecord(h->gfork, st));
+        DCHK(hipStreamWaitEvent(h->xstream[0], h->gfork, 0));
+        DCHK(c2_launch_hist(sc, sl.cand.as<int32_t>(), k.capC, h->opt.xuninf, sl.counts.as<int32_t>(), h->xstream[0],
+                            q.cnt.as<int32_t>() + DL_NC));
+        DCHK(hipEventRecord(h->gjoin[0], h->xstream[0]));
+    }
+    DCHK(c2_launch_phase1q(qa, hp.R, 2, true, P.grid[i], P.lds[i], st));
     DCHK(c2_launch_finalize(sc, sl.cand.as<int32_t>(), k.capC, w.dk, sl.btck.as<uint32_t>(), sl.off.as<int64_t>(),
                             sl.fin.as<FinRec>(), st, q.cnt.as<int32_t>() + DL_NC));
     DCHK(c2_dl_pass(sl.fin.as<FinRec>(), sl.cand.as<int32_t>(), d.len.as<int32_t>(), d.eno.as<float>(),
                     w.scorethld, b, st));
-    DCHK(c2_launch_hist(sc, sl.pass_pro.as<int32_t>(), k.capP, h->opt.xuninf, sl.counts.as<int32_t>(), st,
-                        q.cnt.as<int32_t>() + DL_NPASS));
-    DCHK(c2_launch_stats(P.sp[i], h->dnnw.as<float>(), sl.counts.as<int32_t>(), nullptr, sl.statin.as<StatIn>(),
+    if (ahead) {
+        DCHK(hipStreamWaitEvent(st, h->gjoin[0], 0));
+    } else {
+        DCHK(c2_launch_hist(sc, sl.pass_pro.as<int32_t>(), k.capP, h->opt.xuninf, sl.counts.as<int32_t>(), st,
+                            q.cnt.as<int32_t>() + DL_NPASS));
+    }
+    DCHK(c2_launch_stats(P.sp[i], h->dnnw.as<float>(), sl.counts.as<int32_t>(),
+                         ahead ? sl.passfin.as<int32_t>() : nullptr, sl.statin.as<StatIn>(),
                          k.capP, sl.statout.as<double>(), st, q.cnt.as<int32_t>() + DL_NPASS));
     int mR = 1, mns = 1, mnw = 1;
     c2_map_geometry(hp.Lq, &mR, &mns, &mnw);
@@ -369,7 +386,7 @@ int c2_run_queries_dev(cbp_handle* h, const int32_t* qidx, int32_t nq, const flo
     };
     const std::vector<void*> a0 = addrs();
     for (int i = 0; i < nq; i++) if ((rc = dl_alloc_query(h, *h->dlq[i], P.k))) return rc;
-    for (int s = 0; s < nslots; s++) if ((rc = dl_alloc_slot(h, h->dls[s], P.k, botw))) return rc;
+    for (int s = 0; s < nslots; s++) if ((rc = dl_alloc_slot(h, h->dls[s], P.k, botw, P.split))) return rc;
     if (addrs() != a0) c2_drop_graphs(h);
     /* the backtrack pass's arguments (the device writes their segment count) */
     for (int i = 0; i < nq; i++) {

@@ -306,8 +306,8 @@ __global__ __launch_bounds__(512) void k_map_fwd(ScoreCtx c, MapArgs m)
     load_lse_tab(tab);
     int o0_, n_;
     map_range(m, &o0_, &n_);
-    for (int oi_ = blockIdx.x; oi_ < n_; oi_ += gridDim.x) {
-    const int pi = m.order[o0_ + oi_];
+    if ((int)blockIdx.x >= n_) return;            /* the grid covers the capacity */
+    const int pi = m.order[o0_ + (int)blockIdx.x];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, NW = blockDim.x >> 6;
     const MapPair mp = m.pairs[pi];
     const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63, ns = m.ns;
@@ -416,8 +416,6 @@ __global__ __launch_bounds__(512) void k_map_fwd(ScoreCtx c, MapArgs m)
         for (int k = 1; k < NW; k++) mx = fmx(mx, wmax[k]);
         m.stat[(size_t)pi * 4 + 0] = mx;
     }
-    __syncthreads();
-    }
 }
 
 /* total forward probability in the reference's block order
@@ -444,8 +442,8 @@ __global__ __launch_bounds__(256) void k_map_total(ScoreCtx c, MapArgs m)
     __shared__ float bsum[8];
     int o0_, n_;
     map_range(m, &o0_, &n_);
-    for (int oi_ = blockIdx.x; oi_ < n_; oi_ += gridDim.x) {
-    const int pi = m.order[o0_ + oi_];
+    if ((int)blockIdx.x >= n_) return;            /* the grid covers the capacity */
+    const int pi = m.order[o0_ + (int)blockIdx.x];
     const MapPair mp = m.pairs[pi];
     const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63;
     const float* F = m.F + mp.foff;
@@ -518,8 +516,6 @@ __global__ __launch_bounds__(256) void k_map_total(ScoreCtx c, MapArgs m)
         __syncthreads();
     }
     if (threadIdx.x == 0) m.stat[pi * 4 + 1] = mlog(total) + maxF;   /* logZ */
-    __syncthreads();
-    }
 }
 
 /* ---- backward: B_MM of every cell ----
@@ -536,8 +532,8 @@ __global__ __launch_bounds__(512) void k_map_bwd(ScoreCtx c, MapArgs m)
     load_lse_tab(tab);
     int o0_, n_;
     map_range(m, &o0_, &n_);
-    for (int oi_ = blockIdx.x; oi_ < n_; oi_ += gridDim.x) {
-    const int pi = m.order[o0_ + oi_];
+    if ((int)blockIdx.x >= n_) return;            /* the grid covers the capacity */
+    const int pi = m.order[o0_ + (int)blockIdx.x];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, NW = blockDim.x >> 6;
     const MapPair mp = m.pairs[pi];
     const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63, ns = m.ns;
@@ -646,8 +642,6 @@ __global__ __launch_bounds__(512) void k_map_bwd(ScoreCtx c, MapArgs m)
         }
         __syncthreads();
     }
-    __syncthreads();
-    }
 }
 
 __device__ __forceinline__ bool in_valid(int Lq, int Ld, int e, int y, int x) {
@@ -669,8 +663,8 @@ __global__ __launch_bounds__(512) void k_map_dp(ScoreCtx c, MapArgs m)
     __shared__ int wbx[8], wby[8];
     int o0_, n_;
     map_range(m, &o0_, &n_);
-    for (int oi_ = blockIdx.x; oi_ < n_; oi_ += gridDim.x) {
-    const int pi = m.order[o0_ + oi_];
+    if ((int)blockIdx.x >= n_) return;            /* the grid covers the capacity */
+    const int pi = m.order[o0_ + (int)blockIdx.x];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, NW = blockDim.x >> 6;
     const MapPair mp = m.pairs[pi];
     const int p = mp.pro, Lq = c.Lq, Ld = c.len[p], nsteps = Ld + 63, ns = m.ns;
@@ -779,8 +773,6 @@ __global__ __launch_bounds__(512) void k_map_dp(ScoreCtx c, MapArgs m)
         o[6] = __float_as_int(best);
         o[7] = __float_as_int(logZ);
     }
-    __syncthreads();
-    }
 }
 
 /* ---- backtrace and drawing (FinalizeMAPDP_MAP), one wave per pair ----
@@ -791,8 +783,8 @@ __global__ __launch_bounds__(64) void k_map_final(ScoreCtx c, MapArgs m, int npa
     __shared__ __align__(16) char win[C2_DRAW_WIN];
     int o0_, n_;
     map_range(m, &o0_, &n_);
-    for (int oi_ = blockIdx.x; oi_ < n_; oi_ += gridDim.x) {
-    const int pi = m.order[o0_ + oi_];
+    if ((int)blockIdx.x >= n_) return;            /* the grid covers the capacity */
+    const int pi = m.order[o0_ + (int)blockIdx.x];
     const MapPair mp = m.pairs[pi];
     const int p = mp.pro, Ld = c.len[p];
     int32_t* o = m.res + (size_t)pi * 8;
@@ -804,8 +796,6 @@ __global__ __launch_bounds__(64) void k_map_final(ScoreCtx c, MapArgs m, int npa
     const int stride = (int)m.alnstride;
     c2_walk_draw(c, p, x, y, codes, m.alns + (size_t)pi * 3 * stride, m.alntmp + (size_t)pi * 3 * stride,
                  stride, o);
-    __syncthreads();
-    }
     (void)npairs;
 }
 
@@ -816,11 +806,11 @@ hipError_t launch_r(const ScoreCtx& c, const MapArgs& m0, int npairs, hipStream_
     MapArgs m = m0;
     m.npairs = npairs;
     const int wg = 64 * m.nw;
-    /* with device counts, npairs and nsitems are capacities: the grids are
-     * capped and the kernels loop over the device counts */
+    /* with device counts, npairs and nsitems are capacities: the pair
+     * kernels' grid covers the capacity (blocks past the count exit), the
+     * score items' grid is capped and its waves loop over the items */
     const int sgrid = (m.nsitems + 3) / 4;
     const int grid_s = (m.dcnt && sgrid > 4096) ? 4096 : sgrid;
-    if (m.dcnt && npairs > 4096) npairs = 4096;
     hipError_t e;
     if (m.nsitems > 0)
         hipLaunchKernelGGL((k_map_score<R>), dim3(grid_s), dim3(256), 0, stream, c, m);

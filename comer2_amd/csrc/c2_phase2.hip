@@ -608,8 +608,8 @@ __global__ __launch_bounds__(256) void k_stats(StatParams sp, const float* nnw, 
     __shared__ int sL[4][NUNS], sR[4][NUNS];
     const int wv = threadIdx.x >> 6;
     const int nn = n_dev ? __builtin_amdgcn_readfirstlane(*n_dev) : n;
-    for (int i = blockIdx.x * 4 + wv; i < nn; i += gridDim.x * 4)
-        stats_one(sp, nnw, counts, crow, in, out, i, sP[wv], sNW[wv], spu[wv], sL[wv], sR[wv]);
+    const int i = blockIdx.x * 4 + wv;      /* the grid covers the capacity */
+    if (i < nn) stats_one(sp, nnw, counts, crow, in, out, i, sP[wv], sNW[wv], spu[wv], sL[wv], sR[wv]);
 }
 
 /* S and its SS component for one pair, row-major [Lq][Ldb] */
@@ -663,7 +663,7 @@ hipError_t c2_launch_stats(const StatParams& sp, const float* nnw, const int32_t
                            const StatIn* in, int32_t n, double* out, hipStream_t stream, const int32_t* n_dev)
 {
     if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_stats, dim3(grid_of((n + 3) / 4, n_dev, 2048)), dim3(256), 0, stream, sp, nnw, counts,
+    hipLaunchKernelGGL(k_stats, dim3((n + 3) / 4), dim3(256), 0, stream, sp, nnw, counts,
                        crow, in, n, out, n_dev);
     return hipGetLastError();
 }
