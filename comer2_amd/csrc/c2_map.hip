@@ -297,8 +297,12 @@ __global__ __launch_bounds__(256) void k_map_score(ScoreCtx c, MapArgs m)
  * steps behind wave w-1, so the last row wave w-1 computes in a step is the
  * row above wave w's first row for its next step; it crosses in a
  * double-buffered LDS slot.  Bands hand over through the HBM buffer. */
+/* one row per lane: at most 128 VGPRs, so four waves share a SIMD (the
+ * two-term log-sum-exp table's coefficients otherwise raise the kernel to
+ * 134 and three waves; the 12 bytes it spills per lane cost less than the
+ * lost occupancy: the one-query step fell from 62.4 to 60.7 ms) */
 template <int R>
-__global__ __launch_bounds__(512) void k_map_fwd(ScoreCtx c, MapArgs m)
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(R == 1 ? 4 : 1))) void k_map_fwd(ScoreCtx c, MapArgs m)
 {
     __shared__ float hx[2][8][5];
     __shared__ float wmax[8];
