@@ -14,7 +14,8 @@ class Profiles(C.Structure):
                 ("len", C.POINTER(C.c_int32)), ("dist", C.POINTER(C.c_uint32)),
                 ("trgfrq", C.POINTER(C.c_float)), ("ssprb", C.POINTER(C.c_float)),
                 ("aa", C.POINTER(C.c_char)), ("sss", C.POINTER(C.c_char)),
-                ("trans", C.POINTER(C.c_float))]
+                ("trans", C.POINTER(C.c_float)), ("cv", C.POINTER(C.c_float)),
+                ("cvprior", C.POINTER(C.c_float)), ("cvnorm2", C.POINTER(C.c_float))]
 
 
 class Options(C.Structure):
@@ -22,7 +23,7 @@ class Options(C.Structure):
                 ("evalue", C.c_float), ("ssemodel", C.c_int32), ("xuninf", C.c_int32),
                 ("ung_lambda", C.c_float), ("ung_K", C.c_float), ("ung_H", C.c_float),
                 ("gap_lambda", C.c_float), ("gap_K", C.c_float), ("gap_alpha", C.c_float),
-                ("gap_beta", C.c_float), ("max_map_pairs", C.c_int32)]
+                ("gap_beta", C.c_float), ("max_map_pairs", C.c_int32), ("cvswgt", C.c_float)]
 
 
 class Results(C.Structure):
@@ -47,7 +48,7 @@ NFIELDS = 18
 def options_struct(d):
     o = Options()
     for k, _ in Options._fields_:
-        setattr(o, k, d[k])
+        setattr(o, k, d.get(k, 0.0) if k == "cvswgt" else d[k])
     return o
 
 
@@ -68,6 +69,10 @@ def load_synth():
     lib.c2s_plant.restype = C.c_int32
     lib.c2s_plant.argtypes = [C.c_void_p, C.POINTER(Profiles), C.c_uint64, C.c_int32,
                               C.c_double, C.c_double, C.c_void_p, C.c_void_p]
+    lib.c2s_add_cv.restype = None
+    lib.c2s_add_cv.argtypes = [C.c_void_p, C.c_uint64]
+    lib.c2s_plant_cv.restype = C.c_int32
+    lib.c2s_plant_cv.argtypes = [C.c_void_p, C.POINTER(Profiles), C.c_uint64, C.c_int32]
     return lib
 
 
@@ -84,12 +89,26 @@ class SynthDB:
         self.view = SynthDB._lib.c2s_view(self.h).contents
 
     def plant(self, src, seed, nplant, kmin=4.0, kmax=60.0):
-        """Plant homologous segments of the profiles of `src` (see c2s_plant)."""
+        """Plant homologous segments of the profiles of `src` (see c2s_plant);
+        with context vectors on both sets, the planted windows take the
+        source's vectors (c2s_plant_cv)."""
         ps = np.zeros(nplant, np.int32)
         pp = np.zeros(nplant, np.int32)
         n = SynthDB._lib.c2s_plant(self.h, C.byref(src.view), seed, nplant, kmin, kmax,
                                    ps.ctypes.data_as(C.c_void_p), pp.ctypes.data_as(C.c_void_p))
+        if self.has_cv and src.has_cv:
+            SynthDB._lib.c2s_plant_cv(self.h, C.byref(src.view), seed, nplant)
         return ps[:n], pp[:n]
+
+    def add_cv(self, seed):
+        """Context vectors for every position (see c2s_add_cv)."""
+        SynthDB._lib.c2s_add_cv(self.h, seed)
+        self.view = SynthDB._lib.c2s_view(self.h).contents
+        return self
+
+    @property
+    def has_cv(self):
+        return bool(self.view.cv)
 
     @staticmethod
     def from_lengths(seed, lengths, conc=20.0):

@@ -201,7 +201,7 @@ int dl_chain(cbp_handle* h, int i, const DlPlan& P)
     sa.cap = h->opt.max_map_pairs;
     sa.mapaln = h->opt.mapaln;
     sa.mR = mR; sa.mns = mns;
-    sa.modinuse = h->opt.ssswgt > 0.0f ? 1 : 0;
+    sa.modinuse = (h->opt.ssswgt > 0.0f || h->opt.cvswgt > 0.0f) ? 1 : 0;
     sa.split = P.split ? 1 : 0;
     DCHK(c2_dl_order_select(sl.statout.as<double>(), b, sa, st));
     /* strings are NUL-padded to the stride: the whole buffer is output */

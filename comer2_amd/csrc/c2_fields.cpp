@@ -14,7 +14,9 @@
  * 3 SS probabilities, HDP probability and index, three addresses (int), the
  * residue (char) and the SS state (char).  The fields this path reads are
  * gathered into the field-major arrays of cbp_profiles; the CV, HDP and
- * address fields may be absent (empty) or present, and are not read.
+ * address fields may be absent (empty) or present; the CV fields are read
+ * when present (the context-vector score term), the HDP and address fields
+ * are not.
  */
 #include <cstring>
 #include <vector>
@@ -92,9 +94,29 @@ extern "C" int cbp_fields_pack(const void* const* beg, const void* const* end,
     return CBP_OK;
 }
 
+extern "C" int cbp_fields_pack_cv(const void* const* beg, const void* const* end,
+                                  float* cv, float* cvprior, float* cvnorm2)
+{
+    int32_t np32 = 0;
+    int64_t ns64 = 0;
+    int rc = cbp_fields_count(beg, end, &np32, &ns64);
+    if (rc) return rc;
+    const size_t ns = (size_t)ns64;
+    int present = 0;
+    for (int f = F_CV; f <= F_CVNORM; f++) present += fbytes(beg, end, f) != 0;
+    if (present == 0) return 0;
+    if (present != F_CVNORM - F_CV + 1) return CBP_ERR_ARG;
+    if (!cv || !cvprior || !cvnorm2) return CBP_ERR_ARG;
+    for (int i = 0; i < CBP_NCVEL; i++) std::memcpy(cv + i * ns, beg[F_CV + i], 4 * ns);
+    std::memcpy(cvprior, beg[F_CVPRIOR], 4 * ns);
+    std::memcpy(cvnorm2, beg[F_CVNORM], 4 * ns);
+    return 1;
+}
+
 /* field-major copy of a field set, owned */
 struct FieldSet {
     std::vector<float> bkg, eno, trgfrq, ssprb, trans;
+    std::vector<float> cv, cvprior, cvnorm2;
     std::vector<int32_t> len;
     std::vector<uint32_t> dist;
     std::vector<char> aa, sss;
@@ -109,10 +131,17 @@ struct FieldSet {
         rc = cbp_fields_pack(beg, end, bkg.data(), eno.data(), len.data(), dist.data(), trgfrq.data(),
                              ssprb.data(), aa.data(), sss.data(), trans.data());
         if (rc) return rc;
+        cv.resize((size_t)CBP_NCVEL * ns); cvprior.resize(ns); cvnorm2.resize(ns);
+        rc = cbp_fields_pack_cv(beg, end, cv.data(), cvprior.data(), cvnorm2.data());
+        if (rc < 0) return rc;
+        if (rc == 0) { cv.clear(); cvprior.clear(); cvnorm2.clear(); }
         v.nprof = np; v.npos = ns;
         v.bkg = bkg.data(); v.eno = eno.data(); v.len = len.data(); v.dist = dist.data();
         v.trgfrq = trgfrq.data(); v.ssprb = ssprb.data(); v.aa = aa.data(); v.sss = sss.data();
         v.trans = trans.data();
+        v.cv = cv.empty() ? nullptr : cv.data();
+        v.cvprior = cv.empty() ? nullptr : cvprior.data();
+        v.cvnorm2 = cv.empty() ? nullptr : cvnorm2.data();
         return CBP_OK;
     }
 };
@@ -199,6 +228,25 @@ extern "C" int cbp_upload_db_fields(cbp_handle* h, const void* const* beg1, cons
         c.v.bkg = c.bkg.data(); c.v.eno = c.eno.data(); c.v.len = c.len.data(); c.v.dist = c.dist.data();
         c.v.trgfrq = c.trgfrq.data(); c.v.ssprb = c.ssprb.data(); c.v.aa = c.aa.data(); c.v.sss = c.sss.data();
         c.v.trans = c.trans.data();
+        if (!a.cv.empty() || !b.cv.empty()) {
+            /* a part without context vectors takes the uninformative vector */
+            c.cv.assign((size_t)CBP_NCVEL * ns, C2_CV_UNINF);
+            c.cvprior.assign(ns, 0.0f);
+            c.cvnorm2.assign(ns, 0.0f);
+            for (size_t k = 0; k < (size_t)CBP_NCVEL; k++) {
+                if (!a.cv.empty()) std::copy(a.cv.begin() + k * sa, a.cv.begin() + (k + 1) * sa, c.cv.begin() + k * ns);
+                if (!b.cv.empty()) std::copy(b.cv.begin() + k * sb, b.cv.begin() + (k + 1) * sb, c.cv.begin() + k * ns + sa);
+            }
+            if (!a.cv.empty()) {
+                std::copy(a.cvprior.begin(), a.cvprior.end(), c.cvprior.begin());
+                std::copy(a.cvnorm2.begin(), a.cvnorm2.end(), c.cvnorm2.begin());
+            }
+            if (!b.cv.empty()) {
+                std::copy(b.cvprior.begin(), b.cvprior.end(), c.cvprior.begin() + sa);
+                std::copy(b.cvnorm2.begin(), b.cvnorm2.end(), c.cvnorm2.begin() + sa);
+            }
+            c.v.cv = c.cv.data(); c.v.cvprior = c.cvprior.data(); c.v.cvnorm2 = c.cvnorm2.data();
+        }
         return cbp_upload_chunk(h, &c.v);
     } catch (const std::bad_alloc&) {
         return c2_fail(h, CBP_ERR_MEM, "host memory allocation failed");

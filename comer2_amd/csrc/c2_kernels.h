@@ -8,6 +8,21 @@
 #include <stdint.h>
 #include <stddef.h>
 
+/* the context-vector score term (CalcCVS2ScoreSMEMUnroll2x,
+ * cusco/CuBatchScoreMatrix_cvs2s.cuh:47-136): wgt = CVSWGT, 0 when off */
+struct CvsCtx {
+    const float* cv;      /* DB context vectors [19][npos] */
+    const float* cvp;     /* DB log priors [npos] */
+    const float* cvn;     /* DB squared norms [npos] */
+    const float* qcv;     /* query [21][Lpad]: 19 entries, log prior, squared norm */
+    const float* map;     /* [card] smoothed map scores */
+    int32_t card, shift;
+    int32_t trunc;        /* priors truncated toward zero (MAPALN = 0) */
+    float wgt;
+    float kinv, nu, cterm;      /* 1/KAPPA0, -(NU0+2)/2, CTERM of _TCTXVECT */
+    float k1, v1, kl, vl;       /* the map's first and last key and score */
+};
+
 struct P1Args {
     /* DB chunk (device SoA) */
     const float* trgfrq; const float* ssprb; const float* trans; const float* bkg;
@@ -24,6 +39,7 @@ struct P1Args {
     const float* wts;     /* [nprof][20] 1/(0.5 (qbkg + bkg of the profile)) */
     int32_t Lq, Lpad, nstripes;
     float shift, ssswgt;
+    CvsCtx cvs;
     /* segments of whole profiles */
     const int32_t* seg_pb; /* [nseg] first profile of each segment */
     const int32_t* seg_pe; /* [nseg] one past the last profile */
@@ -56,6 +72,7 @@ struct ScoreCtx {
     const float* qv; const float* QT; const float* qh; const float* qbkg; const char* qaa;
     int32_t Lq, Lpad, R, nstripes;
     float shift, ssswgt;
+    CvsCtx cvs;
 };
 
 /* finalize record of a phase-1 candidate */
@@ -195,6 +212,8 @@ hipError_t c2_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t stre
 /* score weights of the query against every chunk profile into wts [nprof][20] */
 hipError_t c2_launch_weights(const float* qbkg, const float* bkg, int nprof, float* wts, hipStream_t stream);
 hipError_t c2_launch_zero_u32(uint32_t* p, int64_t n, hipStream_t stream);
+/* sets n 32-bit words to v with a kernel */
+hipError_t c2_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t stream);
 /* ncand_dev (or npro_dev, n_dev below): a device count replacing the host
  * count, which then only sizes the grid */
 hipError_t c2_launch_finalize(const ScoreCtx& c, const int32_t* cand, int32_t ncand,

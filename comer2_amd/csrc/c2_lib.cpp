@@ -34,10 +34,20 @@ void HostSet::copy_from(const cbp_profiles* s) {
     aa.assign(s->aa, s->aa + ns);
     sss.assign(s->sss, s->sss + ns);
     trans.assign(s->trans, s->trans + 7 * (ns + np));
+    if (s->cv) {
+        cv.assign(s->cv, s->cv + (size_t)CBP_NCVEL * ns);
+        cvprior.assign(s->cvprior, s->cvprior + ns);
+        cvnorm2.assign(s->cvnorm2, s->cvnorm2 + ns);
+    } else {
+        cv.clear(); cvprior.clear(); cvnorm2.clear();
+    }
     v.nprof = s->nprof; v.npos = s->npos;
     v.bkg = bkg.data(); v.eno = eno.data(); v.len = len.data(); v.dist = dist.data();
     v.trgfrq = trgfrq.data(); v.ssprb = ssprb.data(); v.aa = aa.data(); v.sss = sss.data();
     v.trans = trans.data();
+    v.cv = s->cv ? cv.data() : nullptr;
+    v.cvprior = s->cv ? cvprior.data() : nullptr;
+    v.cvnorm2 = s->cv ? cvnorm2.data() : nullptr;
 }
 
 int c2_dalloc(cbp_handle* h, DBuf& b, size_t n) {
@@ -61,6 +71,9 @@ static int dupload(cbp_handle* h, DBuf& b, const T* src, size_t count) {
 static int check_set(const cbp_profiles* s) {
     if (!s || s->nprof < 1 || s->npos < 1 || !s->bkg || !s->eno || !s->len || !s->dist ||
         !s->trgfrq || !s->ssprb || !s->aa || !s->sss || !s->trans)
+        return 0;
+    /* the three context-vector fields come together or not at all */
+    if ((s->cv == nullptr) != (s->cvprior == nullptr) || (s->cv == nullptr) != (s->cvnorm2 == nullptr))
         return 0;
     int64_t acc = 0;
     for (int32_t p = 0; p < s->nprof; p++) {
@@ -141,8 +154,15 @@ const char* cbp_last_error(const cbp_handle* h)
 int cbp_set_options(cbp_handle* h, const cbp_options* opt)
 {
     if (!h || !opt) return CBP_ERR_ARG;
-    if (!(opt->ssswgt >= 0.0f) || !(opt->gap_lambda > 0.0f) || !(opt->gap_K > 0.0f))
+    if (!(opt->ssswgt >= 0.0f) || !(opt->cvswgt >= 0.0f) || !(opt->gap_lambda > 0.0f) || !(opt->gap_K > 0.0f))
         return fail(h, CBP_ERR_ARG, "invalid options");
+    /* the CV term changes the queries' geometry (at most 2 rows per lane)
+     * and tables: their preparations and the graphs holding them retire */
+    if ((opt->cvswgt > 0.0f) != (h->opt.cvswgt > 0.0f) && !h->qcache.empty()) {
+        (void)hipStreamSynchronize(h->stream);
+        h->qcache.clear();
+        c2_drop_graphs(h);
+    }
     h->opt = *opt;
     h->have_opt = true;
     h->chunk_gen++;
@@ -217,10 +237,28 @@ int c2_append_query(cbp_handle* h, const cbp_profiles* one, int32_t* qidx)
     std::copy(o.dist, o.dist + np, dist.begin()); dist[np] = (uint32_t)ns;
     std::copy(o.aa, o.aa + ns, aa.begin()); std::copy(one->aa, one->aa + L, aa.begin() + ns);
     std::copy(o.sss, o.sss + ns, sss.begin()); std::copy(one->sss, one->sss + L, sss.begin() + ns);
-    cbp_profiles v;
+    cbp_profiles v{};
     v.nprof = (int32_t)np2; v.npos = (int64_t)ns2;
     v.bkg = bkg.data(); v.eno = eno.data(); v.len = len.data(); v.dist = dist.data();
     v.trgfrq = trg.data(); v.ssprb = ss.data(); v.aa = aa.data(); v.sss = sss.data(); v.trans = tr.data();
+    /* context vectors: kept when either part has them, the other part's
+     * positions taking the uninformative vector */
+    std::vector<float> cv, cvp, cvn;
+    if (o.cv || one->cv) {
+        cv.assign((size_t)CBP_NCVEL * ns2, C2_CV_UNINF);
+        cvp.assign(ns2, 0.0f);
+        cvn.assign(ns2, 0.0f);
+        for (size_t i = 0; i < (size_t)CBP_NCVEL; i++) {
+            if (o.cv) std::copy(o.cv + i * ns, o.cv + (i + 1) * ns, cv.begin() + i * ns2);
+            if (one->cv) std::copy(one->cv + i * L, one->cv + (i + 1) * L, cv.begin() + i * ns2 + ns);
+        }
+        if (o.cv) { std::copy(o.cvprior, o.cvprior + ns, cvp.begin()); std::copy(o.cvnorm2, o.cvnorm2 + ns, cvn.begin()); }
+        if (one->cv) {
+            std::copy(one->cvprior, one->cvprior + L, cvp.begin() + ns);
+            std::copy(one->cvnorm2, one->cvnorm2 + L, cvn.begin() + ns);
+        }
+        v.cv = cv.data(); v.cvprior = cvp.data(); v.cvnorm2 = cvn.data();
+    }
     h->queries.copy_from(&v);
     h->chunk_gen++;
     h->sdq_ready = false;
@@ -279,6 +317,14 @@ int cbp_upload_chunk(cbp_handle* h, const cbp_profiles* db)
     if ((rc = dupload(h, d.aa, s.aa, ns))) return rc;
     if ((rc = dupload(h, d.sss, s.sss, ns))) return rc;
     if ((rc = dupload(h, d.trans, s.trans, 7 * (ns + np)))) return rc;
+    d.has_cv = s.cv != nullptr;
+    if (d.has_cv) {
+        if ((rc = dupload(h, d.cv, s.cv, (size_t)CBP_NCVEL * ns))) return rc;
+        if ((rc = dupload(h, d.cvp, s.cvprior, ns))) return rc;
+        if ((rc = dupload(h, d.cvn, s.cvnorm2, ns))) return rc;
+    } else {
+        d.cv.release(); d.cvp.release(); d.cvn.release();
+    }
     /* profile index of every position (the reference's pmv2DAddrPro field) */
     std::vector<int32_t> ap(ns);
     for (int32_t p = 0; p < s.nprof; p++)
@@ -340,8 +386,28 @@ DevQuery* c2_get_query(cbp_handle* h, int qidx, int* rc)
         h->chunk_gen++;
     }
     auto dq = std::make_unique<DevQuery>();
-    c2_prepare_query(&h->queries.v, qidx, h->ssss.data(), h->rmax, &dq->hp);
+    /* the CV term keeps each lane's rows' 21 context-vector values in
+     * registers: at most 2 rows per lane */
+    const bool cvon = h->opt.cvswgt > 0.0f;
+    c2_prepare_query(&h->queries.v, qidx, h->ssss.data(), cvon && h->rmax > 2 ? 2 : h->rmax, &dq->hp);
     const QueryPrep& p = dq->hp;
+    if (cvon) {
+        /* [21][Lpad]: the 19 entries, the log prior, the squared norm; rows
+         * past the query and a query without context vectors take the
+         * uninformative vector */
+        const cbp_profiles& qs = h->queries.v;
+        const size_t Lp = (size_t)p.Lpad, q0 = qs.dist[qidx];
+        std::vector<float> qc(21 * Lp, 0.0f);
+        for (size_t i = 0; i < (size_t)CBP_NCVEL; i++)
+            for (size_t y = 0; y < Lp; y++)
+                qc[i * Lp + y] = (qs.cv && y < (size_t)p.Lq) ? qs.cv[i * (size_t)qs.npos + q0 + y] : C2_CV_UNINF;
+        if (qs.cv)
+            for (size_t y = 0; y < (size_t)p.Lq; y++) {
+                qc[19 * Lp + y] = qs.cvprior[q0 + y];
+                qc[20 * Lp + y] = qs.cvnorm2[q0 + y];
+            }
+        if ((*rc = dupload(h, dq->qcv, qc.data(), qc.size()))) return nullptr;
+    }
     if ((*rc = dupload(h, dq->qv, p.qv.data(), p.qv.size()))) return nullptr;
     if ((*rc = dupload(h, dq->qtr, p.qtr.data(), p.qtr.size()))) return nullptr;
     if ((*rc = dupload(h, dq->qtrb, p.qtrb.data(), p.qtrb.size()))) return nullptr;
@@ -381,7 +447,66 @@ int c2_fill_p1args(cbp_handle* h, DevQuery* dq, P1Args* a)
     a->Lq = dq->hp.Lq; a->Lpad = dq->hp.Lpad; a->nstripes = dq->hp.nstripes;
     a->shift = h->opt.mapaln ? 0.002f : 0.0003f;
     a->ssswgt = h->opt.ssswgt;
+    c2_fill_cvs(h, dq, &a->cvs);
     a->seg_pb = h->dseg_b.as<int32_t>(); a->seg_pe = h->dseg_e.as<int32_t>(); a->nseg = h->nseg;
+    return CBP_OK;
+}
+
+void c2_fill_cvs(cbp_handle* h, DevQuery* dq, CvsCtx* v)
+{
+    std::memset(v, 0, sizeof *v);
+    if (!(h->opt.cvswgt > 0.0f) || !h->have_cvs || !dq) return;
+    const DevSet& d = h->dchunk;
+    v->cv = d.cv.as<float>(); v->cvp = d.cvp.as<float>(); v->cvn = d.cvn.as<float>();
+    v->qcv = dq->qcv.as<float>();
+    v->map = h->dcvmap.as<float>();
+    v->card = (int32_t)h->cvvals.size();
+    v->shift = h->cvshift;
+    v->trunc = h->opt.mapaln ? 0 : 1;
+    v->wgt = h->opt.cvswgt;
+    c2_cv_constants(&v->kinv, &v->nu, &v->cterm);
+    v->k1 = h->cvkeys.front(); v->v1 = h->cvvals.front();
+    v->kl = h->cvkeys.back(); v->vl = h->cvvals.back();
+}
+
+int c2_ensure_chunk_cv(cbp_handle* h)
+{
+    if (!(h->opt.cvswgt > 0.0f)) return CBP_OK;
+    if (!h->have_cvs) return fail(h, CBP_ERR_STATE, "CVSWGT > 0 needs the context-vector map (cbp_upload_cvs2s)");
+    DevSet& d = h->dchunk;
+    if (d.has_cv) return CBP_OK;
+    const size_t ns = (size_t)d.npos;
+    int rc;
+    if ((rc = dalloc(h, d.cv, sizeof(float) * CBP_NCVEL * ns))) return rc;
+    if ((rc = dalloc(h, d.cvp, sizeof(float) * ns))) return rc;
+    if ((rc = dalloc(h, d.cvn, sizeof(float) * ns))) return rc;
+    uint32_t u;
+    std::memcpy(&u, &C2_CV_UNINF, 4);
+    HIPCHK(c2_launch_fill_u32(d.cv.as<uint32_t>(), (int64_t)(CBP_NCVEL * ns), u, h->stream));
+    HIPCHK(c2_launch_fill_u32(d.cvp.as<uint32_t>(), (int64_t)ns, 0u, h->stream));
+    HIPCHK(c2_launch_fill_u32(d.cvn.as<uint32_t>(), (int64_t)ns, 0u, h->stream));
+    HIPCHK(hipStreamSynchronize(h->stream));
+    d.has_cv = true;
+    return CBP_OK;
+}
+
+extern "C" int cbp_upload_cvs2s(cbp_handle* h, const float* keys, const float* values, int32_t card,
+                                int32_t shift)
+{
+    if (!h || !keys || !values) return CBP_ERR_ARG;
+    /* the phase-1 kernels keep the map in 128 words of LDS */
+    if (card < 1 || card > 128 || shift < 0 || shift >= card)
+        return fail(h, CBP_ERR_ARG, "invalid context-vector map (1..128 entries, shift within it)");
+    for (int32_t i = 1; i < card; i++)
+        if (!(keys[i] > keys[i - 1])) return fail(h, CBP_ERR_ARG, "context-vector map keys are not increasing");
+    h->cvkeys.assign(keys, keys + card);
+    h->cvvals.assign(values, values + card);
+    h->cvshift = shift;
+    int rc;
+    if ((rc = dupload(h, h->dcvmap, h->cvvals.data(), (size_t)card))) return rc;
+    HIPCHK(hipStreamSynchronize(h->stream));
+    h->have_cvs = true;
+    h->chunk_gen++;
     return CBP_OK;
 }
 
@@ -396,6 +521,7 @@ int c2_run_phase1(cbp_handle* h, int qidx)
     h->qtick++;
     DevQuery* dq = c2_get_query(h, qidx, &rc);
     if (!dq) return rc;
+    if ((rc = c2_ensure_chunk_cv(h))) return rc;
     P1Args a;
     if ((rc = c2_fill_p1args(h, dq, &a))) return rc;
     a.keys = h->keys.as<unsigned long long>();
@@ -510,6 +636,7 @@ void c2_fill_scorectx(cbp_handle* h, DevQuery* dq, ScoreCtx* c)
     c->Lq = dq->hp.Lq; c->Lpad = dq->hp.Lpad; c->R = dq->hp.R; c->nstripes = dq->hp.nstripes;
     c->shift = h->opt.mapaln ? 0.002f : 0.0003f;
     c->ssswgt = h->opt.ssswgt;
+    c2_fill_cvs(h, dq, &c->cvs);
 }
 
 
@@ -528,6 +655,7 @@ extern "C" int cbp_score_block(cbp_handle* h, int32_t qidx, int32_t pronr, float
     h->qtick++;
     DevQuery* dq = c2_get_query(h, qidx, &rc);
     if (!dq) return rc;
+    if ((rc = c2_ensure_chunk_cv(h))) return rc;
     const int Ld = h->chunk.v.len[pronr];
     const size_t n = (size_t)dq->hp.Lq * Ld;
     DBuf dS, dM;

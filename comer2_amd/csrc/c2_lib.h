@@ -61,6 +61,7 @@ template <class T> using pvec = std::vector<T, PinAlloc<T>>;
 struct HostSet {
     cbp_profiles v{};
     std::vector<float> bkg, eno, trgfrq, ssprb, trans;
+    std::vector<float> cv, cvprior, cvnorm2;     /* empty when the set has no context vectors */
     std::vector<int32_t> len;
     std::vector<uint32_t> dist;
     std::vector<char> aa, sss;
@@ -71,11 +72,17 @@ struct DevSet {
     int32_t nprof = 0;
     int64_t npos = 0;
     DBuf bkg, eno, len, dist, trgfrq, ssprb, aa, sss, trans, addrpro;
+    /* context vectors [19][npos], priors and squared norms [npos]; has_cv
+     * false: none uploaded (filled with the uninformative vector when the
+     * CV term is on) */
+    DBuf cv, cvp, cvn;
+    bool has_cv = false;
 };
 
 struct DevQuery {
     QueryPrep hp;
     DBuf qv, qtr, qtrb, QT, qh, lleft, ldiag, qbkg, qaa, mleft, mdiag;
+    DBuf qcv;            /* [21][Lpad] context vector, prior, squared norm (CV term on) */
 };
 
 /* device and host state of one query's pass through the stage sequence;
@@ -196,6 +203,11 @@ struct cbp_handle {
     std::vector<int32_t> seg_p;
     DBuf dseg_b, dseg_e;
     DBuf dnnw;           /* SS18 parameters on the device */
+    /* context-vector map (cbp_upload_cvs2s) and the model's constants */
+    std::vector<float> cvkeys, cvvals;
+    int32_t cvshift = 0;
+    DBuf dcvmap;
+    bool have_cvs = false;
     /* per-call scratch */
     DBuf cand, cand_off, btck, fin, pass_pro, counts, statin, statout;
     DBuf mpairs, mF, mSd, mhand, mbt, mstat, malns, malntmp, mres;
@@ -293,6 +305,14 @@ void c2_drop_graphs(cbp_handle* h);
  * current device, looked up once per device */
 hipError_t c2_noalloc_init(int device);
 int c2_fill_p1args(cbp_handle* h, DevQuery* dq, P1Args* a);
+/* the CV term's arguments (wgt 0 when the term is off) */
+void c2_fill_cvs(cbp_handle* h, DevQuery* dq, CvsCtx* v);
+/* with the CV term on: the chunk's context vectors (the uninformative
+ * vector when none were uploaded) */
+int c2_ensure_chunk_cv(cbp_handle* h);
+void c2_cv_constants(float* kappa_inv, float* nu_pow, float* cterm);
+/* the uninformative context vector entry of the profile reader */
+constexpr float C2_CV_UNINF = 0.17302947f;
 void c2_fill_scorectx(cbp_handle* h, DevQuery* dq, ScoreCtx* c);
 uint64_t c2_length_adjustment(const cbp_options& o, uint64_t m, uint64_t n, uint64_t N, uint64_t* delta);
 int c2_build_segments(cbp_handle* h, const cbp_profiles* s);

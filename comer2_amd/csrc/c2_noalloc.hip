@@ -58,6 +58,9 @@ extern "C" int cbp_phase1_keys_noalloc(cbp_handle* h, int32_t qidx, uint64_t* ke
     if (!h->have_opt || !h->have_tables || !h->have_queries || !h->have_chunk)
         return c2_fail(h, CBP_ERR_STATE, "options, tables, queries and chunk must be uploaded first");
     if (qidx < 0 || qidx >= h->queries.v.nprof) return c2_fail(h, CBP_ERR_ARG, "query index out of range");
+    /* the scratch layout has no room for the query's context vectors */
+    if (h->opt.cvswgt > 0.0f)
+        return c2_fail(h, CBP_ERR_STATE, "the allocation-free entry point has no context-vector term (CVSWGT > 0)");
     if (hipSetDevice(h->device) != hipSuccess) return c2_fail(h, CBP_ERR_HIP, "hipSetDevice");
     unsigned char* base = nullptr;
     {

@@ -46,6 +46,10 @@ constexpr int BLK = 32;       /* columns loaded per refill */
  * hit 64 distinct banks.  Field offsets within a slot: */
 constexpr int F_DW = 0, F_DT = 20, F_PJ = 27, F_X = 30, F_LEN = 31, F_PID = 32, F_BIN = 33, F_TOP = 34;
 constexpr int NS_BASE = 35, NS_TOP = 39;   /* slot stride without / with the stripe-top fields */
+/* with the context-vector term: the column's 19 entries, its prior as used
+ * and |d|^2 + 1 + 1/kappa after the stripe-top fields */
+constexpr int F_CV = 39, F_CVP = 58, F_CVA = 59, NS_CV = 61;
+constexpr int CV_MAP = 128;                /* LDS words for the CV map */
 
 /* tops: the ring carries the stripe's top boundary; rtop: read it from the
  * hand-off buffer (stripes after the first) instead of the zero row -1 */
@@ -58,7 +62,7 @@ __device__ __forceinline__ int block_profile(PA& a, int lane, int t0, int64_t c_
     return (lane < BLK && c < c_end) ? a.addrpro[c] : -1;
 }
 
-template <int R, class PA>
+template <int R, bool CV, class PA>
 __device__ __forceinline__ void load_block(PA& a, float* ring, int ns, int lane, int t0,
                                            int64_t c_begin, int64_t c_end, bool tops, bool rtop, int p,
                                            const float* bot, int64_t bld, int64_t bc0)
@@ -111,6 +115,13 @@ __device__ __forceinline__ void load_block(PA& a, float* ring, int ns, int lane,
             for (int s = 0; s < 5; s++)
                 rc[F_TOP + s] = rtop ? bt[(size_t)s * bld] : 0.0f;
         }
+        if constexpr (CV) {
+            const float* cv = a.cvs.cv + c;
+            for (int i = 0; i < 19; i++) rc[F_CV + i] = cv[(size_t)i * npos];
+            const float pr = a.cvs.cvp[c];
+            rc[F_CVP] = a.cvs.trunc ? truncf(pr) : pr;
+            rc[F_CVA] = (a.cvs.cvn[c] + 1.0f) + a.cvs.kinv;
+        }
     } else {
         for (int k = 0; k < 20; k++) rc[F_DW + k] = 0.0f;
         for (int k = 0; k < 7; k++) rc[F_DT + k] = -32768.0f;
@@ -121,6 +132,11 @@ __device__ __forceinline__ void load_block(PA& a, float* ring, int ns, int lane,
         rc[F_BIN] = 0.0f;
         if (tops)
             for (int s = 0; s < 5; s++) rc[F_TOP + s] = 0.0f;
+        if constexpr (CV) {
+            for (int i = 0; i < 19; i++) rc[F_CV + i] = 0.0f;
+            rc[F_CVP] = 0.0f;
+            rc[F_CVA] = 1.0f;
+        }
     }
 }
 
@@ -139,7 +155,7 @@ typedef const __attribute__((address_space(4))) P1Args CP1Args;
  * botw != nullptr: the stripe hand-off goes to the wave's own scratch rows
  * of bot_ld columns (indexed from the segment's first column) instead of the
  * query's per-column buffer a.bot */
-template <int R, int NW, bool BTCK, class PA>
+template <int R, int NW, bool BTCK, bool CV, class PA>
 __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw, int64_t bot_ld)
 {
     const int SL = 64 * R;                      /* rows per stripe */
@@ -147,9 +163,15 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
     float* sLL = sQT + 30 * SL;                 /* [5][SL] */
     float* sLD = sLL + 5 * SL;                  /* [5][SL] */
     const bool tops = a.nstripes > 1;
-    const int ns = tops ? NS_TOP : NS_BASE;
+    const int ns = CV ? NS_CV : (tops ? NS_TOP : NS_BASE);
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     float* ring = sLD + 5 * SL + (size_t)wv * ns * RING;
+    /* the CV map after the rings */
+    float* sMap = sLD + 5 * SL + (size_t)NW * ns * RING;
+    if constexpr (CV) {
+        for (int i = threadIdx.x; i < CV_MAP; i += 64 * NW)
+            sMap[i] = i < a.cvs.card ? a.cvs.map[i] : 0.0f;
+    }
 
     for (int s = 0; s < a.nstripes; s++) {
         const int rbase = s * SL;
@@ -192,6 +214,18 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
             for (int j = 0; j < 7; j++) qt[k][j] = a.qtr[(size_t)j * a.Lpad + y0 + k];
             qh[k] = a.qh[y0 + k];
         }
+        /* the rows' context vectors, |q|^2 + 1 + 1/kappa and priors as used */
+        float qc[CV ? R : 1][19], qa[CV ? R : 1], rq[CV ? R : 1];
+        if constexpr (CV) {
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+#pragma unroll
+                for (int i = 0; i < 19; i++) qc[k][i] = a.cvs.qcv[(size_t)i * a.Lpad + y0 + k];
+                const float pr = a.cvs.qcv[(size_t)19 * a.Lpad + y0 + k];
+                rq[k] = a.cvs.trunc ? truncf(pr) : pr;
+                qa[k] = (a.cvs.qcv[(size_t)20 * a.Lpad + y0 + k] + 1.0f) + a.cvs.kinv;
+            }
+        }
         int kf = -1;   /* row k>=1 with r%32==31 (its column-0 diagonal is seen as zeros) */
 #pragma unroll
         for (int k = 1; k < R; k++)
@@ -208,10 +242,15 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
             rc[F_PID] = __int_as_float(-1);
             rc[F_BIN] = 0.0f;
             if (tops) for (int j = 0; j < 5; j++) rc[F_TOP + j] = 0.0f;
+            if constexpr (CV) {
+                for (int j = 0; j < 19; j++) rc[F_CV + j] = 0.0f;
+                rc[F_CVP] = 0.0f;
+                rc[F_CVA] = 1.0f;
+            }
         }
         wave_lds_sync();
         int pn = block_profile(a, lane, 0, c_begin, c_end);
-        load_block<R>(a, ring, ns, lane, 0, c_begin, c_end, tops, s > 0, pn, bot, bld, bc0);
+        load_block<R, CV>(a, ring, ns, lane, 0, c_begin, c_end, tops, s > 0, pn, bot, bld, bc0);
         wave_lds_sync();
 
         S5 H[R];
@@ -238,6 +277,7 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
          * step's refill: the ring then holds the 64 columns the lanes read next
          * and the 32 columns of the block after them */
         float dw[20], dt[7], pj[3];
+        float dc[CV ? 19 : 1], dpr = 0.0f, dna = 0.0f;
         int xloc, nlen, npid;
         uint32_t bb;
         auto read_col = [&]() {
@@ -257,6 +297,12 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
             nlen = __float_as_int(rs[F_LEN]);
             npid = __float_as_int(rs[F_PID]);
             bb = __float_as_uint(rs[F_BIN]);
+            if constexpr (CV) {
+#pragma unroll
+                for (int i = 0; i < 19; i++) dc[i] = rs[F_CV + i];
+                dpr = rs[F_CVP];
+                dna = rs[F_CVA];
+            }
         };
         read_col();
         for (int t = 0; t < nsteps; t++) {
@@ -329,7 +375,26 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
                     ss = fmaf(t2[k], pj[2], ss);
                     ss = fmaf(qh[k], dh, ss);
                     ss = (ss - 0.05f) * ssw;
-                    sc[k] = (sc[k] - a.shift) + ss;
+                    if constexpr (CV) {
+                        /* the CV map score (c2_cvs_x + the map lookup of the
+                         * score kernels), then the terms combined as there */
+                        float dot = qc[k][0] * dc[0];
+#pragma unroll
+                        for (int i = 1; i < 19; i++) dot = fmaf(qc[k][i], dc[i], dot);
+                        dot = dot + a.cvs.kinv;
+                        const float det = fmaf(qa[k], dna, -(dot * dot));
+                        float x = fmaf(a.cvs.nu, c2_logf_clamped(det), a.cvs.cterm);
+                        x = x - rq[k];
+                        x = x - dpr;
+                        const float cm = x <= a.cvs.k1 ? a.cvs.v1
+                                       : (a.cvs.kl <= x ? a.cvs.vl : sMap[a.cvs.shift + (int)rintf(2.0f * x)]);
+                        const float wv = a.cvs.wgt;
+                        sc[k] = sc[k] - a.shift;
+                        if (a.ssswgt > 0.0f) sc[k] = sc[k] + (ss + cm * wv);
+                        else if (cm != 0.0f) sc[k] = sc[k] + cm * wv;
+                    } else {
+                        sc[k] = (sc[k] - a.shift) + ss;
+                    }
                     if ((uint32_t)(u0 - k) >= (uint32_t)cW) sc[k] = -9.0f;   /* distant corners */
                 }
             }
@@ -404,7 +469,7 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
             /* the block after next: its first column is the top boundary the
              * next step reads ahead of its rows */
             if (((t + 2) % BLK) == 0) {
-                load_block<R>(a, ring, ns, lane, t + 2, c_begin, c_end, tops, s > 0,
+                load_block<R, CV>(a, ring, ns, lane, t + 2, c_begin, c_end, tops, s > 0,
                               block_profile(a, lane, t + 2, c_begin, c_end), bot, bld, bc0);
                 wave_lds_sync();
             }
@@ -418,12 +483,12 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
     }
 }
 
-template <int R, int NW, bool BTCK>
+template <int R, int NW, bool BTCK, bool CV = false>
 __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
 {
     extern __shared__ float lds[];
-    p1_sweep<R, NW, BTCK, const P1Args>(a, blockIdx.x * NW + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)),
-                                        lds, nullptr, 0);
+    p1_sweep<R, NW, BTCK, CV, const P1Args>(a, blockIdx.x * NW + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)),
+                                            lds, nullptr, 0);
 }
 
 /* persistent form for a batch of queries: workgroups pull tiles (query,
@@ -450,7 +515,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1q(P1QueueArgs qa)
         /* the arguments are read-only for the whole launch: read through the
          * constant address space, as kernel arguments are */
         CP1Args* ap = (CP1Args*)(qa.args + tq);
-        p1_sweep<R, NW, BTCK, CP1Args>(*ap, tg * NW + wv, lds, botw, qa.bot_ld);
+        p1_sweep<R, NW, BTCK, false, CP1Args>(*ap, tg * NW + wv, lds, botw, qa.bot_ld);
     }
 }
 
@@ -475,6 +540,12 @@ __global__ void k_zero_u32(uint32_t* p, int64_t n)
 {
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
         p[i] = 0u;
+}
+
+__global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
+{
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 }  // namespace
@@ -504,10 +575,40 @@ hipError_t c2_launch_zero_u32(uint32_t* p, int64_t n, hipStream_t stream)
     return hipGetLastError();
 }
 
+hipError_t c2_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    int64_t blocks = (n + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(256), 0, stream, p, n, v);
+    return hipGetLastError();
+}
+
 size_t c2_phase1_lds_bytes(int R, int NW, int nstripes)
 {
     const int ns = nstripes > 1 ? NS_TOP : NS_BASE;
     return sizeof(float) * ((size_t)40 * 64 * R + (size_t)NW * ns * RING);
+}
+
+/* the context-vector variant: rows per lane at most 2 (their 21 CV values
+ * each stay in registers), at most 4 waves per workgroup (a column slot
+ * carries 61 words) */
+static size_t phase1_cv_lds_bytes(int R, int NW)
+{
+    return sizeof(float) * ((size_t)40 * 64 * R + (size_t)NW * NS_CV * RING + CV_MAP);
+}
+
+template <int RR, int NWW, bool BT>
+static hipError_t launch_cv(const P1Args& a, int grid, size_t lds, hipStream_t stream)
+{
+    static bool attr_set = false;
+    if (!attr_set) {
+        (void)hipFuncSetAttribute((const void*)k_phase1<RR, NWW, BT, true>,
+                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+        attr_set = true;
+    }
+    hipLaunchKernelGGL((k_phase1<RR, NWW, BT, true>), dim3(grid), dim3(64 * NWW), lds, stream, a);
+    return hipGetLastError();
 }
 
 #define C2_P1_CASE(RR, NWW, BT)                                                         \
@@ -541,6 +642,14 @@ static hipError_t launch_nw(const P1Args& a, int R, int grid, size_t lds, hipStr
 hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream_t stream)
 {
     if (a.nseg <= 0) return hipSuccess;
+    if (a.cvs.wgt > 0.0f) {
+        if (R < 1 || R > 2 || a.cvs.card < 1 || a.cvs.card > CV_MAP) return hipErrorInvalidValue;
+        const int nw = btck ? 2 : 4;
+        const int grid = (a.nseg + nw - 1) / nw;
+        const size_t lds = phase1_cv_lds_bytes(R, nw);
+        if (btck) return R == 1 ? launch_cv<1, 2, true>(a, grid, lds, stream) : launch_cv<2, 2, true>(a, grid, lds, stream);
+        return R == 1 ? launch_cv<1, 4, false>(a, grid, lds, stream) : launch_cv<2, 4, false>(a, grid, lds, stream);
+    }
     const int grid = (a.nseg + NW - 1) / NW;
     const size_t lds = c2_phase1_lds_bytes(R, NW, a.nstripes);
     if (btck) {

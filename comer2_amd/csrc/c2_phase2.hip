@@ -141,6 +141,7 @@ constexpr int NSC = 641;
 constexpr int HTY = 128;
 constexpr int HQW = 21;
 
+template <bool CV>
 __global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, int xuninf, int32_t* counts,
                                               int32_t npro, const int32_t* npro_dev)
 {
@@ -178,6 +179,14 @@ __global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, in
             co[j] = 10 * j + (int)(pj[j] * 9.999f);
         }
         const float dh = (pj[2] + 0.1f) * 0.1111111f;
+        /* the column's context vector, |d|^2 + 1 + 1/kappa and prior as used */
+        float dcv[CV ? 19 : 1], dca = 0.0f, dcr = 0.0f;
+        if constexpr (CV) {
+#pragma unroll
+            for (int i = 0; i < 19; i++) dcv[i] = c.cvs.cv[(size_t)i * c.npos + col];
+            dca = c2_cvs_norm(c.cvs, c.cvs.cvn[col]);
+            dcr = c2_cvs_prior(c.cvs, c.cvs.cvp[col]);
+        }
         for (int y0 = 0; y0 < c.Lq; y0 += HTY) {
             const int ny = min(HTY, c.Lq - y0);
             __syncthreads();
@@ -199,14 +208,25 @@ __global__ __launch_bounds__(256) void k_hist(ScoreCtx c, const int32_t* pro, in
                 float acc = qr[0] * dw[0];
 #pragma unroll
                 for (int k = 1; k < 20; k++) acc = fmaf(qr[k], dw[k], acc);
-                float sc = c2_logf_clamped(acc) - c.shift;
+                float sc = c2_logf_clamped(acc) - c.shift, ss = 0.0f;
                 if (c.ssswgt > 0.0f) {
-                    float ss = sqt[co[0] * HTY + r] * pj[0];
+                    ss = sqt[co[0] * HTY + r] * pj[0];
                     ss = fmaf(sqt[co[1] * HTY + r], pj[1], ss);
                     ss = fmaf(sqt[co[2] * HTY + r], pj[2], ss);
                     ss = fmaf(qr[20], dh, ss);
                     ss = (ss - 0.05f) * c.ssswgt;
-                    sc = sc + ss;
+                    if (!CV) sc = sc + ss;
+                }
+                if constexpr (CV) {
+                    /* the row's values are the same for the whole wave */
+                    const int yy = y0 + r;
+                    float qc[19];
+#pragma unroll
+                    for (int i = 0; i < 19; i++) qc[i] = c.cvs.qcv[(size_t)i * c.Lpad + yy];
+                    const float qca = c2_cvs_norm(c.cvs, c.cvs.qcv[(size_t)20 * c.Lpad + yy]);
+                    const float qcr = c2_cvs_prior(c.cvs, c.cvs.qcv[(size_t)19 * c.Lpad + yy]);
+                    const float cm = c2_cvs_lookup(c.cvs, c2_cvs_x(c.cvs, qc, qca, qcr, dcv, dca, dcr));
+                    c2_cvs_combine(c.cvs.wgt, c.ssswgt, cm, sc, ss);
                 }
                 if (sc < -10.0f) sc = -10.0f; else if (sc > 10.0f) sc = 10.0f;
                 const int bin = (int)rintf(sc * 32.0f) + 320;
@@ -654,8 +674,12 @@ hipError_t c2_launch_hist(const ScoreCtx& c, const int32_t* pro, int32_t npro, i
                           int32_t* counts, hipStream_t stream, const int32_t* npro_dev)
 {
     if (npro <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_hist, dim3(grid_of(npro, npro_dev, 4096)), dim3(256), 0, stream, c, pro, xuninf, counts,
-                       npro, npro_dev);
+    if (c.cvs.wgt > 0.0f)
+        hipLaunchKernelGGL(k_hist<true>, dim3(grid_of(npro, npro_dev, 4096)), dim3(256), 0, stream, c, pro, xuninf,
+                           counts, npro, npro_dev);
+    else
+        hipLaunchKernelGGL(k_hist<false>, dim3(grid_of(npro, npro_dev, 4096)), dim3(256), 0, stream, c, pro, xuninf,
+                           counts, npro, npro_dev);
     return hipGetLastError();
 }
 

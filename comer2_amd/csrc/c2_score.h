@@ -16,8 +16,72 @@
 #include "c2_dev.h"
 #include "c2_kernels.h"
 
+/* the context-vector map score of a pair of positions
+ * (CalcCVS2ScoreSMEMUnroll2x, cusco/CuBatchScoreMatrix_cvs2s.cuh:47-136, and
+ * GetScoreE1S1Step2Boundary, cupro/SerializedCVS2Scores.cuh:377-410):
+ *   d = q.d + 1/kappa, det = (|q|^2 + 1 + 1/kappa)(|d|^2 + 1 + 1/kappa) - d^2,
+ *   x = CTERM + nu log det - prior_q - prior_d (priors truncated toward zero
+ *   with MAPALN = 0), the map score at key x (the first or last score beyond
+ *   the keys);
+ * qa = |q|^2 + 1 + 1/kappa and da = |d|^2 + 1 + 1/kappa, rq, rd the priors
+ * as used.  A non-positive det (inconsistent norms) takes log(FLT_MIN). */
+__device__ __forceinline__ float c2_cvs_lookup(const CvsCtx& v, float x)
+{
+    if (x <= v.k1) return v.v1;
+    if (v.kl <= x) return v.vl;
+    return v.map[v.shift + (int)rintf(2.0f * x)];
+}
+__device__ __forceinline__ float c2_cvs_prior(const CvsCtx& v, float p) { return v.trunc ? truncf(p) : p; }
+__device__ __forceinline__ float c2_cvs_norm(const CvsCtx& v, float n) { return (n + 1.0f) + v.kinv; }
+__device__ __forceinline__ float c2_cvs_x(const CvsCtx& v, const float* q, float qa, float rq,
+                                          const float* d, float da, float rd)
+{
+    float dot = q[0] * d[0];
+#pragma unroll
+    for (int i = 1; i < 19; i++) dot = fmaf(q[i], d[i], dot);
+    dot = dot + v.kinv;
+    const float det = fmaf(qa, da, -(dot * dot));
+    float x = fmaf(v.nu, c2_logf_clamped(det), v.cterm);
+    x = x - rq;
+    return x - rd;
+}
+/* the term of query row y and chunk column col (map score, unweighted) */
+__device__ __forceinline__ float c2_cvs_cell(const CvsCtx& v, int Lpad, int y, int64_t col, int64_t npos)
+{
+    float q[19], d[19];
+#pragma unroll
+    for (int i = 0; i < 19; i++) {
+        q[i] = v.qcv[(size_t)i * Lpad + y];
+        d[i] = v.cv[(size_t)i * npos + col];
+    }
+    const float qa = c2_cvs_norm(v, v.qcv[(size_t)20 * Lpad + y]);
+    const float rq = c2_cvs_prior(v, v.qcv[(size_t)19 * Lpad + y]);
+    const float da = c2_cvs_norm(v, v.cvn[col]);
+    const float rd = c2_cvs_prior(v, v.cvp[col]);
+    return c2_cvs_lookup(v, c2_cvs_x(v, q, qa, rq, d, da, rd));
+}
+/* adds the CV term to a cell's score: sc enters as log(...) - shift plus
+ * nothing, ss as the weighted SS term (0 when off); on return sc is the
+ * cell's score and ss its 'mod' value.  SS and CV terms together
+ * (CalcSM_SSSS_CVS2S_SMEMUnroll2x, cusco/CuBatchScoreMatrix_ssss_cvs2s.cu:255-271):
+ * t = ss + wgt c, S += t, mod = t - 0.5 wgt.  CV term alone
+ * (CalcSM_CVS2S_SMEMUnroll2x, cusco/CuBatchScoreMatrix_cvs2s.cu:288-299):
+ * for c != 0, S += wgt c and mod = (c - 0.5) wgt (the reference adds this to
+ * a mod buffer nothing initialises; here it starts at 0) */
+__device__ __forceinline__ void c2_cvs_combine(float wgt, float ssswgt, float c, float& sc, float& ss)
+{
+    if (ssswgt > 0.0f) {
+        const float t = ss + c * wgt;
+        sc = sc + t;
+        ss = t - 0.5f * wgt;
+    } else if (c != 0.0f) {
+        sc = sc + c * wgt;
+        ss = (c - 0.5f) * wgt;
+    }
+}
+
 /* score of cell (y, col) of the query vs chunk profile p; *ssout receives the
- * SS component (the reference's 'mod' matrix value for this cell) */
+ * SS and CV component (the reference's 'mod' matrix value for this cell) */
 __device__ __forceinline__ float c2_score_cell(const ScoreCtx& c, int y, int64_t col, int p, float* ssout)
 {
     float acc = 0.0f;
@@ -42,8 +106,10 @@ __device__ __forceinline__ float c2_score_cell(const ScoreCtx& c, int y, int64_t
         ss = fmaf(c.QT[(size_t)co[2] * c.Lpad + y], pj[2], ss);
         ss = fmaf(c.qh[y], dh, ss);
         ss = (ss - 0.05f) * c.ssswgt;
-        sc = sc + ss;
+        if (c.cvs.wgt <= 0.0f) sc = sc + ss;
     }
+    if (c.cvs.wgt > 0.0f)
+        c2_cvs_combine(c.cvs.wgt, c.ssswgt, c2_cvs_cell(c.cvs, c.Lpad, y, col, c.npos), sc, ss);
     *ssout = ss;
     return sc;
 }
@@ -79,8 +145,10 @@ __device__ __forceinline__ float c2_score_cell_w(const ScoreCtx& c, int y, int64
         ss = fmaf(c.QT[(size_t)co[2] * c.Lpad + y], pj[2], ss);
         ss = fmaf(c.qh[y], dh, ss);
         ss = (ss - 0.05f) * c.ssswgt;
-        sc = sc + ss;
+        if (c.cvs.wgt <= 0.0f) sc = sc + ss;
     }
+    if (c.cvs.wgt > 0.0f)
+        c2_cvs_combine(c.cvs.wgt, c.ssswgt, c2_cvs_cell(c.cvs, c.Lpad, y, col, c.npos), sc, ss);
     *ssout = ss;
     return sc;
 }

@@ -45,27 +45,33 @@ struct Chunk { int32_t p0, p1; int64_t pos0, npos; };
 
 /* byte offsets of the fields of a chunk in a staging buffer (256-aligned) */
 struct StageLayout {
-    size_t bkg, eno, len, dist, trg, ss, aa, sss, trans, addr, total;
-    StageLayout(int64_t np, int64_t ns) {
+    size_t bkg, eno, len, dist, trg, ss, aa, sss, trans, addr, cv = 0, cvp = 0, cvn = 0, total;
+    bool has_cv;
+    StageLayout(int64_t np, int64_t ns, bool with_cv) : has_cv(with_cv) {
         size_t o = 0;
         auto take = [&](size_t n) { const size_t r = o; o = (o + n + 255) & ~(size_t)255; return r; };
         bkg = take(4 * 20 * np); eno = take(4 * np); len = take(4 * np); dist = take(4 * np);
         trg = take(4 * 20 * ns); ss = take(4 * 3 * ns); aa = take(ns); sss = take(ns);
         trans = take(4 * 7 * (ns + np)); addr = take(4 * ns);
+        if (with_cv) { cv = take(4 * (size_t)CBP_NCVEL * ns); cvp = take(4 * ns); cvn = take(4 * ns); }
         total = o;
     }
 };
 
 /* device pointers of one chunk set, read by the copy thread */
-struct SetPtrs { char* bkg; char* eno; char* len; char* dist; char* trg; char* ss; char* aa; char* sss; char* trans; char* addr; };
+struct SetPtrs {
+    char* bkg; char* eno; char* len; char* dist; char* trg; char* ss; char* aa; char* sss; char* trans; char* addr;
+    char* cv; char* cvp; char* cvn;
+};
 
 SetPtrs set_ptrs(DevSet& d)
 {
     return SetPtrs{d.bkg.as<char>(), d.eno.as<char>(), d.len.as<char>(), d.dist.as<char>(), d.trgfrq.as<char>(),
-                   d.ssprb.as<char>(), d.aa.as<char>(), d.sss.as<char>(), d.trans.as<char>(), d.addrpro.as<char>()};
+                   d.ssprb.as<char>(), d.aa.as<char>(), d.sss.as<char>(), d.trans.as<char>(), d.addrpro.as<char>(),
+                   d.cv.as<char>(), d.cvp.as<char>(), d.cvn.as<char>()};
 }
 
-int alloc_set(cbp_handle* h, DevSet& d, int64_t np, int64_t ns)
+int alloc_set(cbp_handle* h, DevSet& d, int64_t np, int64_t ns, bool with_cv)
 {
     int rc;
     if ((rc = c2_dalloc(h, d.bkg, 4 * 20 * (size_t)np))) return rc;
@@ -78,6 +84,12 @@ int alloc_set(cbp_handle* h, DevSet& d, int64_t np, int64_t ns)
     if ((rc = c2_dalloc(h, d.sss, (size_t)ns))) return rc;
     if ((rc = c2_dalloc(h, d.trans, 4 * 7 * (size_t)(ns + np)))) return rc;
     if ((rc = c2_dalloc(h, d.addrpro, 4 * (size_t)ns))) return rc;
+    d.has_cv = with_cv;
+    if (with_cv) {
+        if ((rc = c2_dalloc(h, d.cv, 4 * (size_t)CBP_NCVEL * ns))) return rc;
+        if ((rc = c2_dalloc(h, d.cvp, 4 * (size_t)ns))) return rc;
+        if ((rc = c2_dalloc(h, d.cvn, 4 * (size_t)ns))) return rc;
+    }
     return CBP_OK;
 }
 
@@ -104,6 +116,11 @@ void pack_chunk(const cbp_profiles& db, const Chunk& c, char* st, const StageLay
     std::memcpy(st + L.sss, db.sss + c.pos0, ns);
     for (int k = 0; k < 7; k++)
         std::memcpy(st + L.trans + 4 * k * (ns + np), db.trans + k * (NS + NP) + c.pos0 + c.p0, 4 * (ns + np));
+    if (L.has_cv) {
+        for (int i = 0; i < CBP_NCVEL; i++) std::memcpy(st + L.cv + 4 * i * ns, db.cv + i * NS + c.pos0, 4 * ns);
+        std::memcpy(st + L.cvp, db.cvprior + c.pos0, 4 * ns);
+        std::memcpy(st + L.cvn, db.cvnorm2 + c.pos0, 4 * ns);
+    }
 }
 
 hipError_t copy_chunk(const SetPtrs& d, const char* st, const StageLayout& L, int64_t np, int64_t ns, hipStream_t s)
@@ -112,8 +129,11 @@ hipError_t copy_chunk(const SetPtrs& d, const char* st, const StageLayout& L, in
         {d.bkg, L.bkg, 4 * 20 * (size_t)np}, {d.eno, L.eno, 4 * (size_t)np}, {d.len, L.len, 4 * (size_t)np},
         {d.dist, L.dist, 4 * (size_t)np}, {d.trg, L.trg, 4 * 20 * (size_t)ns}, {d.ss, L.ss, 4 * 3 * (size_t)ns},
         {d.aa, L.aa, (size_t)ns}, {d.sss, L.sss, (size_t)ns}, {d.trans, L.trans, 4 * 7 * (size_t)(ns + np)},
-        {d.addr, L.addr, 4 * (size_t)ns}};
+        {d.addr, L.addr, 4 * (size_t)ns},
+        {d.cv, L.cv, L.has_cv ? 4 * (size_t)CBP_NCVEL * ns : 0}, {d.cvp, L.cvp, L.has_cv ? 4 * (size_t)ns : 0},
+        {d.cvn, L.cvn, L.has_cv ? 4 * (size_t)ns : 0}};
     for (auto& p : parts) {
+        if (p.n == 0) continue;
         const hipError_t e = hipMemcpyAsync(p.dst, st + p.off, p.n, hipMemcpyHostToDevice, s);
         if (e != hipSuccess) return e;
     }
@@ -268,6 +288,8 @@ int check_db(const cbp_profiles* s)
     if (!s || s->nprof < 1 || s->npos < 1 || !s->bkg || !s->eno || !s->len || !s->dist ||
         !s->trgfrq || !s->ssprb || !s->aa || !s->sss || !s->trans)
         return 0;
+    if ((s->cv == nullptr) != (s->cvprior == nullptr) || (s->cv == nullptr) != (s->cvnorm2 == nullptr))
+        return 0;
     int64_t acc = 0;
     for (int32_t p = 0; p < s->nprof; p++) {
         if (s->len[p] < 1 || s->len[p] > 65535 || (int64_t)s->dist[p] != acc) return 0;
@@ -329,10 +351,11 @@ int c2_search_db(cbp_handle* h, const cbp_profiles* db, uint64_t chunk_pos, cons
     for (const Chunk& c : ch) { mnp = std::max<int64_t>(mnp, c.p1 - c.p0); mns = std::max(mns, c.npos); }
     int rc;
     /* two device chunk sets and two pinned staging buffers of the largest chunk */
-    if ((rc = alloc_set(h, h->dsets[0], mnp, mns))) return rc;
-    if ((rc = alloc_set(h, h->dsets[1], mnp, mns))) return rc;
+    const bool with_cv = db->cv != nullptr;
+    if ((rc = alloc_set(h, h->dsets[0], mnp, mns, with_cv))) return rc;
+    if ((rc = alloc_set(h, h->dsets[1], mnp, mns, with_cv))) return rc;
     if ((rc = c2_dalloc(h, h->keys, sizeof(unsigned long long) * (size_t)mnp))) return rc;
-    const StageLayout Lmax(mnp, mns);
+    const StageLayout Lmax(mnp, mns, with_cv);
     for (int s = 0; s < 2; s++) {
         if (h->stage_n[s] < Lmax.total) {
             if (h->stage[s]) { (void)hipHostFree(h->stage[s]); h->stage[s] = nullptr; h->stage_n[s] = 0; }
@@ -370,7 +393,7 @@ int c2_search_db(cbp_handle* h, const cbp_profiles* db, uint64_t chunk_pos, cons
             }
             const int s = (int)(k & 1);
             const Chunk& c = ch[k];
-            const StageLayout L(c.p1 - c.p0, c.npos);
+            const StageLayout L(c.p1 - c.p0, c.npos, with_cv);
             /* the staging buffer's previous copies (chunk k-2) have completed:
              * the search of chunk k-2 waited for them */
             pack_chunk(*db, c, stage[s], L);

@@ -15,6 +15,11 @@
  * [minlen, maxlen] (or uniform on it); the database is sorted by length, longest first, as
  * makedb stores it (libpro/srcpro/Db.h:268-275).
  *
+ * Context vectors (optional, c2s_add_cv): 19 entries ~ N(0, 0.5^2) per
+ * position, squared norm = their sum of squares, log prior ~ -U(12, 16);
+ * c2s_plant_cv gives the planted windows the source positions' vectors plus
+ * N(0, 0.1^2) noise, so homologous windows score high in the CV term.
+ *
  * Randomness: Philox4x32-10 keyed by (seed, profile serial), so every profile
  * is generated independently and the result does not depend on the number of
  * OpenMP threads.
@@ -117,6 +122,7 @@ void gen_trans(Philox& g, float t[7]) {
 struct SynthDB {
     cbp_profiles view;
     std::vector<float> bkg, eno, trgfrq, ssprb, trans;
+    std::vector<float> cv, cvprior, cvnorm2;
     std::vector<int32_t> len;
     std::vector<uint32_t> dist;
     std::vector<char> aa, sss;
@@ -297,6 +303,82 @@ int32_t c2s_plant(void* h, const cbp_profiles* src, uint64_t seed, int32_t nplan
     return nplant;
 }
 
+/* Adds context vectors to every position (see the header). */
+void c2s_add_cv(void* h, uint64_t seed)
+{
+    SynthDB* db = (SynthDB*)h;
+    const int32_t np = db->view.nprof;
+    const int64_t npos = db->view.npos;
+    db->cv.assign((size_t)19 * npos, 0.0f);
+    db->cvprior.assign(npos, 0.0f);
+    db->cvnorm2.assign(npos, 0.0f);
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int32_t p = 0; p < np; p++) {
+        Philox r(seed, 0x200000000ull + (uint64_t)p);
+        for (int32_t j = 0; j < db->len[p]; j++) {
+            const int64_t pos = (int64_t)db->dist[p] + j;
+            float n2 = 0.0f;
+            for (int i = 0; i < 19; i++) {
+                const float e = (float)(0.5 * r.normal());
+                db->cv[(size_t)i * npos + pos] = e;
+                n2 += e * e;
+            }
+            db->cvnorm2[pos] = n2;
+            db->cvprior[pos] = (float)(-12.0 - 4.0 * r.uniform());
+        }
+    }
+    cbp_profiles& v = db->view;
+    v.cv = db->cv.data(); v.cvprior = db->cvprior.data(); v.cvnorm2 = db->cvnorm2.data();
+}
+
+/* After c2s_plant with the same seed and source: the planted windows take
+ * the source positions' context vectors plus N(0, 0.1^2) noise (norms
+ * recomputed, priors copied).  Both sets need context vectors. */
+int32_t c2s_plant_cv(void* h, const cbp_profiles* src, uint64_t seed, int32_t nplant)
+{
+    SynthDB* db = (SynthDB*)h;
+    if (!db->view.cv || !src->cv) return -1;
+    const int32_t np = db->view.nprof;
+    if (nplant > np) nplant = np;
+    /* the same choice of profiles and windows as c2s_plant */
+    Philox g(seed, 0xABCDEF0123ull);
+    std::vector<int32_t> idx(np);
+    std::iota(idx.begin(), idx.end(), 0);
+    for (int32_t i = 0; i < nplant; i++) {
+        int32_t j = i + (int32_t)(g.uniform() * (np - i));
+        if (j >= np) j = np - 1;
+        std::swap(idx[i], idx[j]);
+    }
+    std::sort(idx.begin(), idx.begin() + nplant);
+    const int64_t npos = db->view.npos;
+#pragma omp parallel for schedule(dynamic, 8)
+    for (int32_t i = 0; i < nplant; i++) {
+        const int32_t p = idx[i];
+        const int32_t sp = i % src->nprof;
+        Philox r(seed, 0x100000000ull + (uint64_t)p);
+        const int32_t Ld = db->len[p], Ls = src->len[sp];
+        const int32_t m = Ld < Ls ? Ld : Ls;
+        int32_t w = (int32_t)((0.5 + 0.5 * r.uniform()) * m);
+        if (w < 1) w = 1;
+        const int32_t so = (int32_t)(r.uniform() * (Ls - w + 1));
+        const int32_t dof = (int32_t)(r.uniform() * (Ld - w + 1));
+        Philox q(seed, 0x300000000ull + (uint64_t)p);
+        for (int32_t j = 0; j < w; j++) {
+            const int64_t spos = (int64_t)src->dist[sp] + so + j;
+            const int64_t pos = (int64_t)db->dist[p] + dof + j;
+            float n2 = 0.0f;
+            for (int k = 0; k < 19; k++) {
+                const float e = src->cv[(size_t)k * src->npos + spos] + (float)(0.1 * q.normal());
+                db->cv[(size_t)k * npos + pos] = e;
+                n2 += e * e;
+            }
+            db->cvnorm2[pos] = n2;
+            db->cvprior[pos] = src->cvprior[spos];
+        }
+    }
+    return nplant;
+}
+
 /* Copies profiles [p0, p0+n) of a set into a new owned set (dist rebased). */
 void* c2s_subset(void* h, int32_t p0, int32_t n)
 {
@@ -328,6 +410,14 @@ void* c2s_subset(void* h, int32_t p0, int32_t n)
     v.bkg = db->bkg.data(); v.eno = db->eno.data(); v.len = db->len.data();
     v.dist = db->dist.data(); v.trgfrq = db->trgfrq.data(); v.ssprb = db->ssprb.data();
     v.aa = db->aa.data(); v.sss = db->sss.data(); v.trans = db->trans.data();
+    if (a.cv) {
+        db->cv.resize((size_t)19 * npos); db->cvprior.resize(npos); db->cvnorm2.resize(npos);
+        for (int k = 0; k < 19; k++)
+            memcpy(&db->cv[(size_t)k * npos], &a.cv[(size_t)k * a.npos + b0], npos * sizeof(float));
+        memcpy(db->cvprior.data(), a.cvprior + b0, npos * sizeof(float));
+        memcpy(db->cvnorm2.data(), a.cvnorm2 + b0, npos * sizeof(float));
+        v.cv = db->cv.data(); v.cvprior = db->cvprior.data(); v.cvnorm2 = db->cvnorm2.data();
+    }
     return db;
 }
 

@@ -42,6 +42,7 @@ def lib():
     L.cbp_last_error.restype = C.c_char_p
     L.cbp_set_options.argtypes = [H, C.POINTER(Options)]
     L.cbp_upload_tables.argtypes = [H, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32]
+    L.cbp_upload_cvs2s.argtypes = [H, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_int32, C.c_int32]
     L.cbp_upload_queries.argtypes = [H, C.POINTER(Profiles)]
     L.cbp_upload_chunk.argtypes = [H, C.POINTER(Profiles)]
     L.cbp_set_db_details.argtypes = [H, C.c_uint64, C.c_uint64]
@@ -78,7 +79,7 @@ def _fptr(a):
 class CuBatchProcessing:
     """One device worker (the reference constructs one per DevCommThread)."""
 
-    def __init__(self, device=0, mem_budget=0, area=0, options=None, ssss=None, nnw=None):
+    def __init__(self, device=0, mem_budget=0, area=0, options=None, ssss=None, nnw=None, cvmap=None):
         from . import tables
         self.L = lib()
         self.h = C.c_void_p()
@@ -91,6 +92,10 @@ class CuBatchProcessing:
         ss = np.ascontiguousarray(tables.load_ssss() if ssss is None else ssss, np.float32)
         nw = np.ascontiguousarray(tables.load_nn_weights() if nnw is None else nnw, np.float32)
         self._check(self.L.cbp_upload_tables(self.h, _fptr(ss), _fptr(nw), nw.size))
+        keys, vals, shift = tables.load_cvs2s() if cvmap is None else cvmap
+        self._cvk = np.ascontiguousarray(keys, np.float32)
+        self._cvv = np.ascontiguousarray(vals, np.float32)
+        self._check(self.L.cbp_upload_cvs2s(self.h, _fptr(self._cvk), _fptr(self._cvv), self._cvk.size, int(shift)))
         self.scorethld = None
         self.logevthld = None
         self.dblen = 0

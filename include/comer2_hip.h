@@ -48,7 +48,14 @@ enum {
  *   aa     [npos]           residue letters (pmv2Daa)
  *   sss    [npos]           SS state (pmv2DSSstate)
  *   trans  [7][npos+nprof]  log transitions MM MI MD IM II DM DD, len+1 rows
- *                           per profile, row 0 = beginning (ptr2DTrnPrbs)  */
+ *                           per profile, row 0 = beginning (ptr2DTrnPrbs)
+ *   cv     [19][npos]       context vectors (pmv2DCVentrs)
+ *   cvprior [npos]          log prior of the context vector (pmv2DCVprior)
+ *   cvnorm2 [npos]          squared norm of the context vector (pmv2DCVnorm2)
+ * cv, cvprior and cvnorm2 are NULL together when the profiles carry no
+ * context vectors; the CV score term then sees the reader's uninformative
+ * vector (every entry 0.17302947, prior 0, norm 0;
+ * cupro/IOProfileModel.cu:700-707) */
 typedef struct {
     int32_t         nprof;
     int64_t         npos;
@@ -61,7 +68,13 @@ typedef struct {
     const char*     aa;
     const char*     sss;
     const float*    trans;
+    const float*    cv;
+    const float*    cvprior;
+    const float*    cvnorm2;
 } cbp_profiles;
+
+/* number of context-vector entries per position (pmv2DNoCVEls) */
+#define CBP_NCVEL 19
 
 /* options (src/env/options.txt) and statistical parameters (src/env/comer.par) */
 typedef struct {
@@ -74,6 +87,8 @@ typedef struct {
     float   ung_lambda, ung_K, ung_H;             /* [UNGAPPED]  comer.par:5-11 */
     float   gap_lambda, gap_K, gap_alpha, gap_beta; /* [GAPPED_A_1] comer.par:15-20 */
     int32_t max_map_pairs; /* cap on realigned pairs per query (<=0: no cap) */
+    float   cvswgt;     /* CVSWGT  options.txt:74; > 0 adds the context-vector
+                           score term (needs cbp_upload_cvs2s) */
 } cbp_options;
 
 /* per-query search outcome; buffers are owned by the handle and stay valid
@@ -114,6 +129,22 @@ const char* cbp_last_error(const cbp_handle* h);
 int  cbp_set_options(cbp_handle* h, const cbp_options* opt);
 int  cbp_upload_tables(cbp_handle* h, const float* ssss900, const float* nnw, int32_t nnw_count);
 
+/* the context-vector score map (src/env/cvs2s) as CVS2Scores reads it
+ * (libpro/srcpro/CVS2Scores.cpp:54-466): the first table's keys and scores,
+ * the scores smoothed by the centred five-point moving average of
+ * SmoothScores, missing keys inserted at step 1/2, shift = number of
+ * negative keys.  Host only, no GPU.  keys and values hold cap entries;
+ * *card receives the table size */
+int  cbp_cvs2s_read(const char* path, float* keys, float* values, int32_t cap,
+                    int32_t* card, int32_t* shift);
+/* upload the map for the CV score term (CuDeviceMemory::CacheCVS2ScoresDevice,
+ * cupro/CuDeviceMemory.cu:671-723): a score x maps to values[0] when
+ * x <= keys[0], to values[card-1] when x >= keys[card-1], else to
+ * values[shift + rint(2x)] (SerializedCVS2Scores::GetScoreE1S1Step2Boundary,
+ * cupro/SerializedCVS2Scores.cuh:377-410) */
+int  cbp_upload_cvs2s(cbp_handle* h, const float* keys, const float* values, int32_t card,
+                      int32_t shift);
+
 /* data: queries (CuDeviceMemory::CacheData / CacheDataDevice,
  * cupro/CuDeviceMemory.cu:234-250, 1105-1141); DB chunk
  * (CuBatchProcessing::TransferCPMData -> CuDeviceMemory::TransferCPMData /
@@ -140,6 +171,12 @@ int  cbp_fields_count(const void* const* beg, const void* const* end, int32_t* n
 int  cbp_fields_pack(const void* const* beg, const void* const* end,
                      float* bkg, float* eno, int32_t* len, uint32_t* dist,
                      float* trgfrq, float* ssprb, char* aa, char* sss, float* trans);
+/* the context-vector fields of a field set, cv [19][npos], cvprior and
+ * cvnorm2 [npos]; returns 1 when the set carries them (every CV field spans
+ * npos values), 0 when the CV fields are empty (nothing written), <0 on
+ * error (host only, no GPU) */
+int  cbp_fields_pack_cv(const void* const* beg, const void* const* end,
+                        float* cv, float* cvprior, float* cvnorm2);
 /* cbp_upload_chunk of a field set (CuBatchProcessing::TransferCPMData,
  * cupro/CuBatchProcessing.cuh:60-68 -> CuDeviceMemory::CopyCPMDataToDevice,
  * cupro/CuDeviceMemory.cu:1036-1104) */
@@ -311,7 +348,8 @@ int  cbp_sdp_prepare(cbp_handle* h, const cbp_sdp_pair* pairs, int32_t npairs, i
                      int64_t* ws_bytes);
 /* score matrices of the prepared pairs into S (device), the score of
  * CalcSM_Init_SMEMUnroll2x + CalcSM_SSSS_SMEMUnroll2x from the raw profile
- * fields (cusco/CuBatchScoreMatrix_Init.cu:45-186, CuBatchScoreMatrix_ssss.cu:51-280) */
+ * fields (cusco/CuBatchScoreMatrix_Init.cu:45-186, CuBatchScoreMatrix_ssss.cu:51-280);
+ * the context-vector term is not added here (a caller adds it to S) */
 int  cbp_sdp_scores(cbp_handle* h, float* S, void* stream);
 /* logZ[npairs] (natural log) */
 int  cbp_sdp_forward(cbp_handle* h, const float* S, const float* theta, float* logZ,

@@ -26,9 +26,23 @@ extern "C" {
 typedef struct {
     float ssss[30][30];     /* SS score table (src/env/ssss), symmetric */
     float nnw[93];          /* SS18 weights, CuDeviceMemory.cu:40-80 */
+    /* context-vector map (src/env/cvs2s after CVS2Scores' smoothing):
+     * keys, scores, size, number of negative keys; used when CVSWGT > 0 */
+    float cvkeys[128], cvvals[128];
+    int32_t cvcard, cvshift;
 } or_tables;
 
 /* --- scores (cusco) --- */
+/* the context-vector model's constants of _TCTXVECT
+ * (libpro/srcpro/TCTXVECT.cpp:20-54): 1/KAPPA0, PowerNU0, CTERM */
+void or_cv_constants(float* kinv, float* nu, float* cterm);
+/* the CV map score of query position qpos of q against position dpos of d
+ * (CalcCVS2ScoreSMEMUnroll2x, cusco/CuBatchScoreMatrix_cvs2s.cuh:47-136,
+ * GetScoreE1S1Step2Boundary, cupro/SerializedCVS2Scores.cuh:377-410), FP32
+ * in the kernels' order; trunc: priors truncated toward zero (MAPALN 0);
+ * profiles without context vectors take the uninformative vector */
+float or_cvs_score(const cbp_profiles* q, int64_t qpos, const cbp_profiles* d, int64_t dpos,
+                   const or_tables* t, int trunc);
 void or_score_f32(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
                   const or_tables* t, const cbp_options* o, float* S, float* mod);
 void or_score_f64(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,

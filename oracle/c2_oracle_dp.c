@@ -39,6 +39,46 @@ static inline int ssidx(int s, float p) { return s * 10 + (int)(p * 9.999f); }
  *   w_a = 1/(0.5(bq_a+bd_a)), d'_a = d_a w_a, acc = fma chain over a = 0..19,
  *   S = log(acc) - shift;  QT_r[g] = sum_i p_i T[idx_i][g] (fma chain),
  *   ss = sum_j p'_j QT_r[idx_j] + (p_H+0.1)((p'_H+0.1)*0.1111111), ss = (ss-0.05)*w */
+void or_cv_constants(float* kinv, float* nu, float* cterm)
+{
+    /* _TCTXVECT: DIM 19, KAPPA0 0.1, NU0 = DIM, loKAPPA0 = 1/KAPPA0,
+     * PowerNU0 = -0.5 (NU0 + 2), CTERM = (0.5 DIM + PowerNU0)(ln KAPPA0 -
+     * ln(KAPPA0 + 2)) + sum_{i=1..DIM} (ln(NU0 + 1 - i) - ln 2), in float */
+    const float kappa = 0.1f, dimf = 19.0f;
+    float pw = -0.5f * (dimf + 2.0f);
+    float ct = (0.5f * 19 + pw) * (logf(kappa) - logf(kappa + 2.0f));
+    float acc = 0.0f;
+    for (int i = 1; i <= 19; i++) acc += logf(dimf + 1.0f - (float)i) - 0.69314718055994530942f;
+    *kinv = 1.0f / kappa; *nu = pw; *cterm = ct + acc;
+}
+
+static float cv_at(const cbp_profiles* s, int64_t pos, int i) {
+    return s->cv ? s->cv[(size_t)i * s->npos + pos] : 0.17302947f;
+}
+
+float or_cvs_score(const cbp_profiles* q, int64_t qpos, const cbp_profiles* d, int64_t dpos,
+                   const or_tables* t, int trunc)
+{
+    float kinv, nu, cterm;
+    or_cv_constants(&kinv, &nu, &cterm);
+    /* dot product of the two vectors plus the off-diagonal 1/kappa */
+    float dot = cv_at(q, qpos, 0) * cv_at(d, dpos, 0);
+    for (int i = 1; i < 19; i++) dot = fmaf(cv_at(q, qpos, i), cv_at(d, dpos, i), dot);
+    dot = dot + kinv;
+    const float nq = q->cv ? q->cvnorm2[qpos] : 0.0f, nd = d->cv ? d->cvnorm2[dpos] : 0.0f;
+    float pq = q->cv ? q->cvprior[qpos] : 0.0f, pd = d->cv ? d->cvprior[dpos] : 0.0f;
+    if (trunc) { pq = (float)(int)pq; pd = (float)(int)pd; }
+    /* det(C_2 + H'H) of the shifted centring matrix */
+    const float det = fmaf((nq + 1.0f) + kinv, (nd + 1.0f) + kinv, -(dot * dot));
+    float x = fmaf(nu, c2_logf_clamped(det), cterm);
+    x = x - pq;
+    x = x - pd;
+    const int card = t->cvcard;
+    if (x <= t->cvkeys[0]) return t->cvvals[0];
+    if (t->cvkeys[card - 1] <= x) return t->cvvals[card - 1];
+    return t->cvvals[t->cvshift + (int)rintf(2.0f * x)];
+}
+
 void or_score_f32(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
                   const or_tables* t, const cbp_options* o, float* S, float* mod)
 {
@@ -90,7 +130,20 @@ void or_score_f32(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
                 ss = fmaf(qt[cj[c * 3 + 2]], pj[c * 3 + 2], ss);
                 ss = fmaf(qh[r], dh[c], ss);
                 ss = (ss - 0.05f) * o->ssswgt;
-                s = s + ss;
+                if (!(o->cvswgt > 0.0f)) s = s + ss;
+            }
+            if (o->cvswgt > 0.0f) {
+                /* CalcSM_SSSS_CVS2S_SMEMUnroll2x (cusco/CuBatchScoreMatrix_ssss_cvs2s.cu:255-271)
+                 * and CalcSM_CVS2S_SMEMUnroll2x (cusco/CuBatchScoreMatrix_cvs2s.cu:288-299) */
+                const float cm = or_cvs_score(q, (int64_t)q0 + r, d, (int64_t)d0 + c, t, !o->mapaln);
+                if (o->ssswgt > 0.0f) {
+                    const float tot = ss + cm * o->cvswgt;
+                    s = s + tot;
+                    ss = tot - 0.5f * o->cvswgt;
+                } else if (cm != 0.0f) {
+                    s = s + cm * o->cvswgt;
+                    ss = (cm - 0.5f) * o->cvswgt;
+                }
             }
             S[(size_t)r * Ld + c] = s;
             if (mod) mod[(size_t)r * Ld + c] = ss;
@@ -129,6 +182,18 @@ void or_score_f64(const cbp_profiles* q, int qp, const cbp_profiles* d, int dp,
                       ((double)d->ssprb[(size_t)2 * d->npos + d0 + c] + 0.1) * (double)0.1111111f;
                 ss = (ss - (double)0.05f) * (double)o->ssswgt;
                 s += ss;
+            }
+            if (o->cvswgt > 0.0f) {
+                /* the map lookup is a step function of its key: the FP32
+                 * key of the kernels is used here too */
+                const double cm = (double)or_cvs_score(q, (int64_t)q0 + r, d, (int64_t)d0 + c, t, !o->mapaln);
+                if (o->ssswgt > 0.0f) {
+                    s += cm * (double)o->cvswgt;
+                    ss = ss + cm * (double)o->cvswgt - 0.5 * (double)o->cvswgt;
+                } else if (cm != 0.0) {
+                    s += cm * (double)o->cvswgt;
+                    ss = (cm - 0.5) * (double)o->cvswgt;
+                }
             }
             S[(size_t)r * Ld + c] = s;
             if (mod) mod[(size_t)r * Ld + c] = ss;

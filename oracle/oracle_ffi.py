@@ -13,7 +13,9 @@ LIB = os.path.join(ROOT, "oracle", "liboracle.so")
 
 
 class Tables(C.Structure):
-    _fields_ = [("ssss", (C.c_float * 30) * 30), ("nnw", C.c_float * 93)]
+    _fields_ = [("ssss", (C.c_float * 30) * 30), ("nnw", C.c_float * 93),
+                ("cvkeys", C.c_float * 128), ("cvvals", C.c_float * 128),
+                ("cvcard", C.c_int32), ("cvshift", C.c_int32)]
 
 
 class DpMax(C.Structure):
@@ -62,21 +64,29 @@ def lib():
     return _lib
 
 
-def make_tables(ssss, nnw):
+def make_tables(ssss, nnw, cvmap=None):
+    """cvmap: (keys, values, shift) of the context-vector map, or None"""
     t = Tables()
     arr = np.ctypeslib.as_array(t.ssss)
     arr[:] = ssss
     np.ctypeslib.as_array(t.nnw)[:] = nnw
+    if cvmap is not None:
+        keys, vals, shift = cvmap
+        n = len(keys)
+        np.ctypeslib.as_array(t.cvkeys)[:n] = keys
+        np.ctypeslib.as_array(t.cvvals)[:n] = vals
+        t.cvcard = n
+        t.cvshift = shift
     return t
 
 
 class Oracle:
-    def __init__(self, ssss, nnw, opts):
+    def __init__(self, ssss, nnw, opts, cvmap=None):
         self.L = lib()
-        self.t = make_tables(ssss, nnw)
+        self.t = make_tables(ssss, nnw, cvmap)
         self.o = Options()
         for k, _ in Options._fields_:
-            setattr(self.o, k, opts[k])
+            setattr(self.o, k, opts.get(k, 0.0) if k == "cvswgt" else opts[k])
 
     def score_f32(self, q, qp, d, dp):
         Lq, Ld = q.len[qp], d.len[dp]

@@ -7,7 +7,8 @@ from oracle_ffi import Oracle
 
 
 def oracle(opts=None):
-    return Oracle(tables.load_ssss(), tables.load_nn_weights(), dict(tables.default_options(), **(opts or {})))
+    return Oracle(tables.load_ssss(), tables.load_nn_weights(), dict(tables.default_options(), **(opts or {})),
+                  tables.load_cvs2s())
 
 
 def expected_keys(orc, q, qp, d):
