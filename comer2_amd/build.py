@@ -13,7 +13,7 @@ HIPCC = os.path.join(ROCM, "bin", "hipcc")
 
 HIP_SOURCES = ["c2_phase1.hip", "c2_phase2.hip", "c2_map.hip", "c2_sdp.hip", "c2_lib.cpp", "c2_pipe.cpp",
                "c2_prep.cpp", "c2_fields.cpp", "c2_batchproc.cpp", "c2_stream.cpp",
-               "c2_noalloc.hip", "c2_dlist.hip", "c2_dpipe.cpp", "c2_cvs.cpp"]
+               "c2_noalloc.hip", "c2_dlist.hip", "c2_dpipe.cpp", "c2_cvs.cpp", "c2_proread.cpp", "c2_format.cpp"]
 HIP_FLAGS = ["--offload-arch=gfx950", "-O3", "-std=c++17", "-fPIC",
              "-ffp-contract=off", "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
              "-Wno-unused-command-line-argument"]

@@ -625,6 +625,16 @@ uint64_t c2_length_adjustment(const cbp_options& o, uint64_t m, uint64_t n, uint
     return (uint64_t)sspace;
 }
 
+extern "C" int cbp_search_space(const cbp_options* opt, uint64_t qlen, uint64_t dblen, uint64_t nseqs,
+                                uint64_t* sspace, int32_t* deltalen)
+{
+    if (!opt || !sspace || !deltalen) return CBP_ERR_ARG;
+    uint64_t d = 0;
+    *sspace = c2_length_adjustment(*opt, qlen, dblen, nseqs, &d);
+    *deltalen = (int32_t)d;
+    return CBP_OK;
+}
+
 void c2_fill_scorectx(cbp_handle* h, DevQuery* dq, ScoreCtx* c)
 {
     DevSet& d = h->dchunk;

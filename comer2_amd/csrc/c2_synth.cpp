@@ -5,7 +5,8 @@
  *
  * Per position: 20 target probabilities ~ Dirichlet(conc * bg) with bg the
  * Robinson background (libpro/srcpro/SUBSTABLE.cpp:404-406), residue =
- * argmax, 3 SS probabilities ~ Dirichlet(1,1,1), SS state = argmax, and the
+ * argmax, 3 SS probabilities ~ Dirichlet(1,1,1), SS state = argmax (stored as
+ * the reader stores it: ' ', 'e', 'h' for C, E, H), and the
  * 7 used transitions (MM, MI, MD, IM, II, DM, DD) drawn row-wise: M-row ~
  * Dirichlet(18,1,1) (E[MM] = 0.9), I-row (IM,II) and D-row (DM,DD) ~
  * Dirichlet(3,2).  Transitions are stored as logs, with IM, II, DM, DD scaled
@@ -221,7 +222,7 @@ static void* generate_from(uint64_t seed, int32_t nprof, const std::vector<int32
                 db->ssprb[(size_t)s * npos + pos] = (float)ss[s];
                 if (ss[s] > ss[sm]) sm = s;
             }
-            db->sss[pos] = (char)sm;
+            db->sss[pos] = " eh"[sm];
             gen_trans(g, t);
             for (int k = 0; k < 7; k++) db->trans[(size_t)k * ntr + pos + p + 1] = t[k];
         }
@@ -295,7 +296,7 @@ int32_t c2s_plant(void* h, const cbp_profiles* src, uint64_t seed, int32_t nplan
                 db->ssprb[(size_t)s * npos + pos] = (float)s3[s];
                 if (s3[s] > s3[sm]) sm = s;
             }
-            db->sss[pos] = (char)sm;
+            db->sss[pos] = " eh"[sm];
         }
         if (plant_src) plant_src[i] = sp;
         if (plant_pro) plant_pro[i] = p;

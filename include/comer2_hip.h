@@ -194,6 +194,30 @@ int  cbp_upload_db_fields(cbp_handle* h, const void* const* beg1, const void* co
  * CuBatchProcessing::ProcessScoreMatrix, cupro/CuBatchProcessing.cuh:82-94) */
 int  cbp_add_query_fields(cbp_handle* h, const void* const* beg, uint64_t nqyposs, int32_t* qidx);
 
+/* COMER text profiles ("COMER profile v2.0", as makepro writes them:
+ * TextWriteProfileCondensed, libpro/srcpro/PMProfileModel.cpp:2295-2510)
+ * read into a profile set the library owns (TextReadProfileHeader and
+ * TextReadProfileData, cupro/IOProfileModel.cu:46-1021).  Host only, no
+ * GPU.  A file holds one profile or several one after another; every
+ * cbp_proset_read_text call appends its file's profiles.  The set's
+ * cbp_profiles (valid until the next read or the free) feeds
+ * cbp_upload_queries / cbp_upload_chunk; descriptions and file names are
+ * kept per profile for the output. */
+typedef struct cbp_proset cbp_proset;
+int  cbp_proset_create(cbp_proset** out);
+int  cbp_proset_read_text(cbp_proset* s, const char* path);
+/* a binary profile database, format v2.2 ("COMER profile bin database
+ * v2.2": Db::MakeBin_v2_2, libpro/srcpro/Db.cpp:333-383, read as
+ * TdDataReader::OpenAndReadPreamble, cupro/TdDataReader.cpp:385-467, and the
+ * chunk reads after it do), into an empty set: its sections are the
+ * field-major arrays of cbp_profiles; context vectors always come with it */
+int  cbp_proset_read_bin(cbp_proset* s, const char* path);
+const cbp_profiles* cbp_proset_profiles(const cbp_proset* s);
+const char* cbp_proset_desc(const cbp_proset* s, int32_t i);
+const char* cbp_proset_name(const cbp_proset* s, int32_t i);
+const char* cbp_proset_error(const cbp_proset* s);
+void cbp_proset_free(cbp_proset* s);
+
 /* whole-database size used for thresholds and E-values
  * (CuBatchProcessing::SetDbDetails) */
 int  cbp_set_db_details(cbp_handle* h, uint64_t dblen, uint64_t nseqs);
@@ -238,6 +262,48 @@ int  cbp_search_db(cbp_handle* h, const cbp_profiles* db, uint64_t chunk_pos,
                    const int32_t* qidx, int32_t nq, cbp_results* outs /* [nq] */);
 /* number of chunks of the last cbp_search_db call */
 int  cbp_last_search_chunks(const cbp_handle* h, int32_t* nchunks);
+
+/* a query's results as the reference's plain-text output file (host only,
+ * no GPU): the programme header, the query, the database lines, one
+ * annotation per hit, the alignment records and the summary
+ * (AlnWriter::WriteResultsPlain, cualn/AlnWriterPlain.cpp:19-121, with the
+ * records of CuBatchProcessingFinalizer::CompressResultsPlain,
+ * cupro/CuBatchProcessingFinalizerPlain.cpp:31-143).  Hits are the pairs
+ * with log E <= res->logevthld and an alignment drawn, in res->order.  The
+ * DB profile arrays are indexed by the records' profile index
+ * (CBP_F_ORGPRONO).  out may be NULL to get the length; otherwise cap must
+ * exceed it (the text is NUL-terminated). */
+typedef struct {
+    const char* progname;          /* programme name and version line (NULL: none) */
+    const char* progversion;
+    const char* const* references; /* NULL-terminated reference lines, or NULL */
+    const char* query_desc;
+    int32_t     query_len;
+    float       query_eno;
+    const char* query_sss;         /* [query_len] SS states (' ', 'e', 'h') or NULL */
+    const char* const* db_names;   /* database names */
+    int32_t     n_db_names;
+    uint64_t    db_size;           /* total positions of the database */
+    uint64_t    db_entries;        /* profiles of the database */
+    const char* const* pro_desc;   /* per DB profile: description */
+    const int32_t* pro_len;        /* per DB profile: length */
+    const float* pro_eno;          /* per DB profile: ENO */
+    const uint32_t* pro_dist;      /* per DB profile: first position (for pro_sss) */
+    const char* pro_sss;           /* SS states of the DB positions, or NULL */
+    int32_t     deltalen;          /* length adjustment (cbp_search_space) */
+    int32_t     nohits, noalns;    /* NOHITS, NOALNS (options.txt:11,13; < 0: all) */
+    int32_t     dsclen, dscwidth, alnwidth;   /* DSCLEN, DSCWIDTH, ALNWIDTH (:18,20,22) */
+    int32_t     show;              /* SHOW (:15): the statistics footer of each record */
+} cbp_plain_info;
+int  cbp_format_plain(const cbp_options* opt, const cbp_plain_info* info, const cbp_results* res,
+                      char* out, size_t cap, size_t* len);
+
+/* effective search space and length adjustment of a query of qlen
+ * positions against dblen positions in nseqs profiles
+ * (AbstractScoreMatrix::ComputeLengthAdjustment,
+ * libpro/srcsco/AbstractScoreMatrix.cpp:247-390); host only */
+int  cbp_search_space(const cbp_options* opt, uint64_t qlen, uint64_t dblen, uint64_t nseqs,
+                      uint64_t* sspace, int32_t* deltalen);
 
 /* ---- stage entry points (exposed for parity tests and profiling) ---- */
 

@@ -56,7 +56,7 @@ or_search_out or_search(const cbp_profiles* q, int qp, const cbp_profiles* d,
                 csum += st.Evalue;
                 if (o->mapaln && st.Evalue <= logevthld) {
                     const or_map r = or_map_direct(q, qp, d, dp, S, M, (float)st.EpA, o->minpp,
-                                                   o->ssswgt > 0.0f, NULL, NULL, bt, NULL);
+                                                   o->ssswgt > 0.0f || o->cvswgt > 0.0f, NULL, NULL, bt, NULL);
                     nmap++;
                     mcells += (int64_t)n;
                     csum += r.alnlen;

@@ -230,7 +230,9 @@ def _oracle_map(self, q, qp, d, dp, S, M, epa, direct=True, want=True):
     F = np.zeros((Lq, Ld), np.float32) if want else None
     FB = np.zeros((Lq, Ld), np.float32) if want else None
     bt = np.zeros((Lq, Ld), np.int8) if want else None
-    modin = 1 if self.o.ssswgt > 0 else 0
+    # the mod scores are in use with either term (GetModScoreMatrixInUse,
+    # cupro/CuDeviceMemory.cuh:142-144)
+    modin = 1 if (self.o.ssswgt > 0 or self.o.cvswgt > 0) else 0
     if direct:
         aln = np.zeros((3, Lq + Ld + 1), np.uint8) if want else None
         r = L.or_map_direct(C.byref(q), qp, C.byref(d), dp, np.ascontiguousarray(S, np.float32),
@@ -249,7 +251,9 @@ def _oracle_map64(self, q, qp, d, dp, S64, M64, epa):
         _map_sigs(L)
         L._map_ready = True
     post = np.zeros((q.len[qp], d.len[dp]), np.float64)
-    modin = 1 if self.o.ssswgt > 0 else 0
+    # the mod scores are in use with either term (GetModScoreMatrixInUse,
+    # cupro/CuDeviceMemory.cuh:142-144)
+    modin = 1 if (self.o.ssswgt > 0 or self.o.cvswgt > 0) else 0
     lz = L.or_map_f64(C.byref(q), qp, C.byref(d), dp, np.ascontiguousarray(S64), np.ascontiguousarray(M64),
                       epa, modin, _ptr(post))
     return lz, post

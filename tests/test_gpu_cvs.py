@@ -12,7 +12,8 @@ covers a 5-stripe query of one row per lane.
 * phase-1 maxima of every profile bit-exact;
 * the whole pipeline (one-query calls and a batch) equal to the oracle's
   hit lists, records and strings, MAPALN 1 (MAP) and 0 (phase-1 drawing),
-  and with a DB without context vectors (the uninformative vector);
+  with a DB without context vectors (the uninformative vector), and with the
+  SS term off (the MAP's dynamic score then takes the CV term's mod);
 * a whole-DB search in three chunks equal to the one-chunk search.
 """
 import numpy as np
@@ -75,17 +76,17 @@ def test_cv_phase1_keys_bitexact():
     assert not np.array_equal(plain.phase1_keys(2), cbp.phase1_keys(2))
 
 
-@pytest.mark.parametrize("mapaln,db_cv", [(1, True), (0, True), (1, False)])
-def test_cv_pipeline_matches_oracle(mapaln, db_cv):
+@pytest.mark.parametrize("mapaln,db_cv,ssswgt", [(1, True, 0.12), (0, True, 0.12), (1, False, 0.12), (1, True, 0.0)])
+def test_cv_pipeline_matches_oracle(mapaln, db_cv, ssswgt):
     q, d = _sets(db_cv)
-    opts = dict(cvswgt=0.15, mapaln=mapaln, max_map_pairs=12)
+    opts = dict(cvswgt=0.15, mapaln=mapaln, max_map_pairs=12, ssswgt=ssswgt)
     orc = oracle(opts)
     cbp = _cbp(q, d, opts)
     thld, logev = 6.0, float(np.log(1e-2))
     drawn = 0
     one = []
     for qp in range(len(QLENS)):
-        print(f"cv pipeline: mapaln={mapaln} db_cv={db_cv} query {qp}", flush=True)
+        print(f"cv pipeline: mapaln={mapaln} db_cv={db_cv} ssswgt={ssswgt} query {qp}", flush=True)
         res = cbp.ProcessScoreMatrix(qp, thld, logev)
         exp = search_expect(orc, q, qp, d, thld, logev, d.npos, d.nprof, 12)
         assert_search_equal(res, exp, ("cv", mapaln, db_cv, qp))
