@@ -84,6 +84,100 @@ void fmt_desc(std::string& o, const char* desc, int maxoutlen, int indent, int w
 
 uint32_t bits(float f) { uint32_t u; std::memcpy(&u, &f, 4); return u; }
 
+/* the hits: the pairs under the log E threshold with an alignment drawn, in
+ * the results' order; drawn_at maps a record to its row of strings */
+struct Hits {
+    std::vector<int> list;
+    std::vector<int> drawn_at;
+};
+
+void collect_hits(const cbp_results* res, Hits& h)
+{
+    h.drawn_at.assign(res->npassed > 0 ? res->npassed : 0, -1);
+    for (int j = 0; j < res->nmap; j++)
+        if (res->map_rows[j] >= 0 && res->map_rows[j] < res->npassed) h.drawn_at[res->map_rows[j]] = j;
+    for (int i = 0; i < res->npassed; i++) {
+        const int k = res->order[i];
+        const float* r = res->alndata + (size_t)k * CBP_NFIELDS;
+        if (res->logevthld < r[CBP_F_EVALUE]) continue;
+        if ((int32_t)bits(r[CBP_F_EPA]) < 1) continue;
+        h.list.push_back(k);
+    }
+}
+
+/* the query, middle and subject strings of record k; false if it has none
+ * or they are shorter than its alignment length */
+bool record_rows(const cbp_results* res, const Hits& h, int k, int alnlen, const char*& rq, const char*& rm,
+                 const char*& rt)
+{
+    const int j = h.drawn_at[k];
+    if (j < 0 || !res->alns) return false;
+    rq = res->alns + ((size_t)j * 3 + 0) * res->alnstride;
+    rm = res->alns + ((size_t)j * 3 + 1) * res->alnstride;
+    rt = res->alns + ((size_t)j * 3 + 2) * res->alnstride;
+    return (int)strnlen(rq, (size_t)res->alnstride) >= alnlen;
+}
+
+/* SS rows along an alignment: a gap takes ' ', a residue its position's
+ * state (cumapdp/CuBatchMAPDP_map_final.cu:249-300) */
+void ss_rows(const cbp_plain_info* in, uint32_t pro, unsigned qrybeg, unsigned trgbeg, const char* rq,
+             const char* rt, int alnlen, std::string& qss, std::string& tss)
+{
+    const size_t t0 = (size_t)in->pro_dist[pro];
+    size_t qy = qrybeg - 1, tx = trgbeg - 1;
+    for (int c = 0; c < alnlen; c++) {
+        qss += rq[c] == '-' ? ' ' : in->query_sss[qy];
+        tss += rt[c] == '-' ? ' ' : in->pro_sss[t0 + tx];
+        if (rq[c] != '-') qy++;
+        if (rt[c] != '-') tx++;
+    }
+}
+
+double evalue_of(float le) { return kLogDpMax < le ? 1.0e+308 : exp((double)le); }
+
+/* bit score and P-value of a record (GetBitScore, GetPvalue) */
+float bit_score(float sspace, float logeval) { return sspace <= 0.0f ? 0.0f : (logf(sspace) - logeval) / kLn2; }
+double pvalue(double evalue) { return evalue < 0.01 ? evalue : 1.0 - exp(-evalue); }
+
+/* FormatDescriptionJSON (liblib/fmtdescription.h:96-126): leading blanks
+ * dropped, runs of blanks one space, '"', '\\' and '/' escaped by a
+ * backslash; at maxoutlen the last three characters, and backslashes just
+ * before them, become '.' */
+void fmt_desc_json(std::string& o, const char* desc, int maxoutlen, int& outpos)
+{
+    while (*desc == ' ' || *desc == '\t' || *desc == '\n' || *desc == '\r') desc++;
+    while (*desc && outpos < maxoutlen) {
+        if (*desc == ' ' || *desc == '\t') {
+            while (*desc == ' ' || *desc == '\t') desc++;
+            o += ' '; outpos++;
+            continue;
+        }
+        if (*desc == '"' || *desc == '\\' || *desc == '/') {
+            o += '\\'; outpos++;
+            if (maxoutlen <= outpos) break;
+        }
+        o += *desc++; outpos++;
+    }
+    if (maxoutlen <= outpos && 3 < maxoutlen) {
+        /* the reference steps back over the buffer it wrote, outpos
+         * characters of which may precede this call's text */
+        size_t i = 1;
+        for (; i <= 3 && i <= o.size(); i++) o[o.size() - i] = '.';
+        for (; i <= (size_t)outpos && i <= o.size() && o[o.size() - i] == '\\'; i++) o[o.size() - i] = '.';
+    }
+}
+
+int finish(const std::string& o, char* out, size_t cap, size_t* len)
+{
+    *len = o.size();
+    if (out) {
+        if (cap < o.size() + 1) return CBP_ERR_ARG;
+        std::memcpy(out, o.data(), o.size());
+        out[o.size()] = 0;
+    }
+    return CBP_OK;
+}
+
 }  // namespace
 
 extern "C" int cbp_format_plain(const cbp_options* opt, const cbp_plain_info* in, const cbp_results* res,
@@ -128,17 +222,9 @@ extern "C" int cbp_format_plain(const cbp_options* opt, const cbp_plain_info* in
             }
         }
         /* the hits: drawn pairs under the threshold, in the sorted order */
-        std::vector<int> hits;
-        std::vector<int> drawn_at(res->npassed > 0 ? res->npassed : 0, -1);
-        for (int j = 0; j < res->nmap; j++)
-            if (res->map_rows[j] >= 0 && res->map_rows[j] < res->npassed) drawn_at[res->map_rows[j]] = j;
-        for (int i = 0; i < res->npassed; i++) {
-            const int k = res->order[i];
-            const float* r = res->alndata + (size_t)k * CBP_NFIELDS;
-            if (res->logevthld < r[CBP_F_EVALUE]) continue;
-            if ((int32_t)bits(r[CBP_F_EPA]) < 1) continue;
-            hits.push_back(k);
-        }
+        Hits H;
+        collect_hits(res, H);
+        const std::vector<int>& hits = H.list;
         /* database lines and the heading of the hit list (limited to the
          * writer's 1 KB scratch, with "..." when it runs out) */
         {
@@ -186,7 +272,6 @@ extern "C" int cbp_format_plain(const cbp_options* opt, const cbp_plain_info* in
             o += s;
         }
         const float sspace = (float)res->search_space;
-        auto evalue_of = [](float le) { return kLogDpMax < le ? 1.0e+308 : exp((double)le); };
         if (!hits.empty()) {
             /* annotations */
             for (size_t i = 0; i < hits.size() && (in->nohits < 0 || i < (size_t)in->nohits); i++) {
@@ -221,9 +306,8 @@ extern "C" int cbp_format_plain(const cbp_options* opt, const cbp_plain_info* in
                 const int psts = (int)r[CBP_F_PSTVS], idts = (int)r[CBP_F_IDNTS], gaps = (int)r[CBP_F_NGAPS];
                 put(o, "  Length/ENO: Query = %d/%.1f, Sbjct = %d/%.1f\n\n", in->query_len, in->query_eno, dbprolen,
                     in->pro_eno[pro]);
-                const float bitsc = sspace <= 0.0f ? 0.0f : (logf(sspace) - logeval) / kLn2;
-                const double pval = evalue < 0.01 ? evalue : 1.0 - exp(-evalue);
-                put(o, " Score = %.2f (%.1f bits),  Expect = %.2g, P-value = %.2g\n", score, bitsc, evalue, pval);
+                put(o, " Score = %.2f (%.1f bits),  Expect = %.2g, P-value = %.2g\n", score, bit_score(sspace, logeval),
+                    evalue, pvalue(evalue));
                 if (idts > 0) put(o, " Identities = %d/%d (%d%%)", idts, alnlen, idts * 100 / alnlen);
                 if (psts > 0) {
                     if (idts) o += ",";
@@ -235,26 +319,12 @@ extern "C" int cbp_format_plain(const cbp_options* opt, const cbp_plain_info* in
                 }
                 o += "\n\n";
                 /* the alignment (FormatAlignmentPlain) */
-                const int j = drawn_at[k];
-                if (j < 0 || !res->alns) return CBP_ERR_ARG;
-                const char* rq = res->alns + ((size_t)j * 3 + 0) * res->alnstride;
-                const char* rm = res->alns + ((size_t)j * 3 + 1) * res->alnstride;
-                const char* rt = res->alns + ((size_t)j * 3 + 2) * res->alnstride;
-                if ((int)strnlen(rq, (size_t)res->alnstride) < alnlen) return CBP_ERR_ARG;
+                const char *rq, *rm, *rt;
+                if (!record_rows(res, H, k, alnlen, rq, rm, rt)) return CBP_ERR_ARG;
                 const uint32_t bc = bits(r[CBP_F_BEGCOORDS]);
                 unsigned qrybeg = (bc & 0xFFFFu) + 1, trgbeg = (bc >> 16) + 1;
-                /* SS rows along the alignment: a gap takes ' ' */
                 std::string qss, tss;
-                if (printsss) {
-                    const size_t q0 = 0, t0 = (size_t)in->pro_dist[pro];
-                    size_t qy = qrybeg - 1, tx = trgbeg - 1;
-                    for (int c = 0; c < alnlen; c++) {
-                        qss += rq[c] == '-' ? ' ' : in->query_sss[q0 + qy];
-                        tss += rt[c] == '-' ? ' ' : in->pro_sss[t0 + tx];
-                        if (rq[c] != '-') qy++;
-                        if (rt[c] != '-') tx++;
-                    }
-                }
+                if (printsss) ss_rows(in, pro, qrybeg, trgbeg, rq, rt, alnlen, qss, tss);
                 const int aw = in->alnwidth;
                 for (int f = 0; f < alnlen; f += aw) {
                     int nbytes = alnlen - f;
@@ -321,11 +391,213 @@ extern "C" int cbp_format_plain(const cbp_options* opt, const cbp_plain_info* in
     } catch (const std::bad_alloc&) {
         return CBP_ERR_MEM;
     }
-    *len = o.size();
-    if (out) {
-        if (cap < o.size() + 1) return CBP_ERR_ARG;
-        std::memcpy(out, o.data(), o.size());
-        out[o.size()] = 0;
+    return finish(o, out, cap, len);
+}
+
+/* The JSON output file of a query (AlnWriter::WriteResultsJSON,
+ * cualn/AlnWriterJSON.cpp:20-143): the programme and references
+ * (WritePrognameJSON, :151-197), the query (WriteQueryDescriptionJSON,
+ * :210-265), the database and message (WriteSearchInformationJSON,
+ * :280-360), the summary entries (MakeAnnotationJSON,
+ * cupro/CuBatchProcessingFinalizerJSON.cpp:165-195), the hit records
+ * (CompressResultsJSON, FormatScoresJSON, FormatAlignmentJSON,
+ * FormatFooterJSON, :35-382) and the search statistics (WriteSummaryJSON,
+ * cualn/AlnWriterJSON.cpp:375-427).  The records' separators follow the
+ * writer: a record is followed by ",\n" unless it is the last of the hits
+ * or of the first NOHITS (for the records as well as the summary entries,
+ * so NOALNS < NOHITS leaves a trailing comma, as the reference does). */
+extern "C" int cbp_format_json(const cbp_options* opt, const cbp_plain_info* in, const cbp_results* res,
+                               char* out, size_t cap, size_t* len)
+{
+    if (!opt || !in || !res || !len || in->query_len < 1 || in->dsclen < 1) return CBP_ERR_ARG;
+    if (res->npassed > 0 && (!res->alndata || !res->order)) return CBP_ERR_ARG;
+    if (res->npassed > 0 && (!in->pro_desc || !in->pro_len || !in->pro_eno)) return CBP_ERR_ARG;
+    std::string o;
+    try {
+        /* the programme and its references: written when 1 KB for them
+         * fits the writer's buffer, at most 964 bytes of reference text */
+        {
+            put(o, "{\"comer_search\": {\n  \"program\": \"%s\",\n  \"version\": \"%s\",\n",
+                in->progname ? in->progname : "", in->progversion ? in->progversion : "");
+            const int maxsize = kWriterBuffer - (int)o.size() - 1024;
+            if (maxsize >= 1) {
+                static const char* const none[1] = {nullptr};
+                const char* const* refs = in->references ? in->references : none;
+                o += "  \"references\": [\n";
+                int totlen = 0;
+                bool bwrt = true;
+                for (int i = 0; !i || (bwrt && refs[i]); i++) {
+                    const char* r = refs[i];
+                    totlen += r ? (int)std::strlen(r) : 0;
+                    bwrt = totlen < 1024 - 60;
+                    put(o, "    \"%s\"%s\n", (bwrt && r) ? r : "", (bwrt && r && refs[i + 1]) ? "," : "");
+                }
+                o += "  ],\n";
+            }
+        }
+        /* the query */
+        {
+            int left = kWriterBuffer - (int)o.size();
+            if (left > in->dsclen) left = in->dsclen;
+            if (left >= 60) {
+                const size_t b = o.size();
+                put(o, "  \"query\": {\n    \"length\": %d,\n", in->query_len);
+                const int maxsize = left - (int)(o.size() - b) - 256;
+                if (maxsize >= 1) {
+                    o += "    \"name\": \"\",\n    \"description\": \"";
+                    int outpos = 36;
+                    if (in->query_desc) fmt_desc_json(o, in->query_desc, maxsize, outpos);
+                    o += "\"\n  },\n";
+                }
+            }
+        }
+        Hits H;
+        collect_hits(res, H);
+        const std::vector<int>& hits = H.list;
+        /* the database and the message, within the writer's 1 KB scratch */
+        {
+            int maxsize = kTmpBuffer;
+            auto lfn = [&maxsize](int d) { maxsize -= d; return maxsize < 1; };
+            if (!lfn(40 * 7)) {
+                o += "  \"database\": {\n    \"names\": [\n";
+                for (int i = 0; i < in->n_db_names; i++) {
+                    const char* nm = in->db_names && in->db_names[i] ? in->db_names[i] : "";
+                    size_t sz = std::strlen(nm);
+                    if (99 < sz) sz = 99;
+                    if (lfn((int)sz + 15)) { o += "      \"...\"\n"; break; }
+                    o += "      \"";
+                    int outpos = 0;
+                    fmt_desc_json(o, nm, (int)sz + 1, outpos);
+                    o += i + 1 < in->n_db_names ? "\",\n" : "\"\n";
+                }
+                put(o, "    ],\n    \"number_of_profiles\": %zu,\n    \"number_of_positions\": %zu\n  },\n",
+                    (size_t)in->db_entries, (size_t)in->db_size);
+                if (!lfn(90 * 4)) {
+                    if (!hits.empty())
+                        o += "  \"message\": \"Profiles found below the e-value threshold:\",\n";
+                    else
+                        put(o, "  \"message\": \"No profiles found below an e-value threshold of %g\",\n",
+                            exp((double)res->logevthld));
+                }
+            }
+        }
+        const size_t nh = in->nohits < 0 ? hits.size() : std::min(hits.size(), (size_t)in->nohits);
+        const size_t na = in->noalns < 0 ? hits.size() : std::min(hits.size(), (size_t)in->noalns);
+        auto sep = [&](size_t i) { return (i + 1 < hits.size() && i + 1 < nh) ? ",\n" : "\n"; };
+        /* the summary entries */
+        o += "  \"search_summary\": [\n";
+        for (size_t i = 0; i < nh; i++) {
+            const float* r = res->alndata + (size_t)hits[i] * CBP_NFIELDS;
+            const uint32_t pro = bits(r[CBP_F_ORGPRONO]);
+            o += "    {\"summary_entry\": {\n      \"description\": \"";
+            int outpos = 0;
+            if (in->pro_desc[pro]) fmt_desc_json(o, in->pro_desc[pro], kAnnotLen, outpos);
+            put(o, "\",\n      \"score\": %.0f,\n      \"evalue\": %.2g\n    }}", r[CBP_F_SCORE],
+                evalue_of(r[CBP_F_EVALUE]));
+            o += sep(i);
+        }
+        o += "  ],\n  \"search_hits\": [\n";
+        /* the hit records */
+        const float sspace = (float)res->search_space;
+        const bool qsss = in->query_sss != nullptr;
+        for (size_t i = 0; i < na; i++) {
+            const int k = hits[i];
+            const float* r = res->alndata + (size_t)k * CBP_NFIELDS;
+            const uint32_t pro = bits(r[CBP_F_ORGPRONO]);
+            const int alnlen = (int32_t)bits(r[CBP_F_EPA]);
+            const float logeval = r[CBP_F_EVALUE];
+            const double evalue = evalue_of(logeval);
+            const float score = r[CBP_F_SCORE];
+            o += "    {\"hit_record\": {\n      \"target_description\": \"";
+            {
+                int outpos = 0;
+                if (in->pro_desc[pro]) fmt_desc_json(o, in->pro_desc[pro], in->dsclen, outpos);
+            }
+            put(o, "\",\n      \"query_length\": %d,\n      \"query_eno\": %.1f,\n      \"target_length\": %d,\n"
+                   "      \"target_eno\": %.1f,\n      \"alignment\": {\n",
+                in->query_len, in->query_eno, in->pro_len[pro], in->pro_eno[pro]);
+            /* scores (FormatScoresJSON) */
+            put(o, "        \"score\": %.2f,\n        \"bit_score\": %.1f,\n        \"evalue\": %.2g,\n"
+                   "        \"pvalue\": %.2g,\n",
+                score, bit_score(sspace, logeval), evalue, pvalue(evalue));
+            put(o, "        \"n_identities\": %d,\n        \"n_positives\": %d,\n        \"n_gaps\": %d,\n"
+                   "        \"aln_length\": %d,\n",
+                (int)r[CBP_F_IDNTS], (int)r[CBP_F_PSTVS], (int)r[CBP_F_NGAPS], alnlen);
+            /* the alignment (FormatAlignmentJSON) */
+            const char *rq, *rm, *rt;
+            if (!record_rows(res, H, k, alnlen, rq, rm, rt)) return CBP_ERR_ARG;
+            const uint32_t bc = bits(r[CBP_F_BEGCOORDS]);
+            const unsigned qrybeg = (bc & 0xFFFFu) + 1, trgbeg = (bc >> 16) + 1;
+            const int qg = (int)std::count(rq, rq + alnlen, '-'), tg = (int)std::count(rt, rt + alnlen, '-');
+            const unsigned qryend = qrybeg + alnlen - qg, trgend = trgbeg + alnlen - tg;
+            /* the flags say whether both carry SS states; the states are
+             * drawn along the alignment only under SSSWGT > 0
+             * (cumapdp/CuBatchMAPDP.cu:440), so the strings are empty
+             * otherwise */
+            const bool printsss = qsss && in->pro_sss;
+            const bool drawsss = printsss && opt->ssswgt > 0.0f;
+            put(o, "        \"query_from\": %u,\n        \"query_to\": %u,\n        \"target_from\": %u,\n"
+                   "        \"target_to\": %u,\n",
+                qrybeg, alnlen <= qg ? qryend : qryend - 1, trgbeg, alnlen <= tg ? trgend : trgend - 1);
+            put(o, "        \"query_secstr_available\": %d,\n        \"target_secstr_available\": %d,\n"
+                   "        \"query_secstr\": \"",
+                (int)printsss, (int)printsss);
+            std::string qss, tss;
+            if (drawsss) ss_rows(in, pro, qrybeg, trgbeg, rq, rt, alnlen, qss, tss);
+            o += qss;
+            o += "\",\n        \"target_secstr\": \"";
+            o += tss;
+            o += "\",\n        \"query_aln\": \"";
+            o.append(rq, alnlen);
+            o += "\",\n        \"target_aln\": \"";
+            o.append(rt, alnlen);
+            o += "\",\n        \"middle\": \"";
+            o.append(rm, alnlen);
+            o += "\",\n";
+            /* statistics (FormatFooterJSON) */
+            {
+                const float K = r[CBP_F_K], lmbd = r[CBP_F_LMBD], Ke = r[CBP_F_KEST], le = r[CBP_F_LMBDEST];
+                const float expected = r[CBP_F_E], entropy = r[CBP_F_H];
+                char kb[64], lb[64];
+                const char* kp = "NA";
+                const char* lp = "NA";
+                if (expected < 0.0f) {
+                    if (K > 0.0f) { std::snprintf(kb, sizeof kb, "%6.4f", K); kp = kb; }
+                    if (lmbd > 0.0f) { std::snprintf(lb, sizeof lb, "%6.4f", lmbd); lp = lb; }
+                }
+                put(o, "        \"computed_K_ungapped\": \"%s\",\n        \"computed_lambda_ungapped\": \"%s\",\n", kp,
+                    lp);
+                kp = lp = "NA";
+                if (expected < 0.0f) {
+                    if (Ke > 0.0f) { std::snprintf(kb, sizeof kb, "%6.4f", Ke); kp = kb; }
+                    if (le > 0.0f) { std::snprintf(lb, sizeof lb, "%6.4f", le); lp = lb; }
+                }
+                put(o, "        \"estimated_K_gapped\": \"%s\",\n        \"estimated_lambda_gapped\": \"%s\",\n", kp,
+                    lp);
+                put(o, "        \"entropy\": %.4f,\n        \"expected_score\": %.4f,\n        \"min_score\": %.0f,\n"
+                       "        \"max_score\": %.0f\n",
+                    entropy, expected, r[CBP_F_MIN], r[CBP_F_MAX]);
+            }
+            o += "      }\n    }}";
+            o += sep(i);
+        }
+        o += "  ],\n";
+        /* search statistics (WriteSummaryJSON) */
+        {
+            int effq = in->query_len;
+            uint64_t effdb = in->db_size;
+            if (in->deltalen < in->query_len) effq -= in->deltalen;
+            if (in->db_entries * (uint64_t)in->deltalen < in->db_size) effdb -= in->db_entries * (uint64_t)in->deltalen;
+            put(o, "  \"search_statistics\": {\n    \"reference_K_ungapped\": %.4f,\n"
+                   "    \"reference_lambda_ungapped\": %.4f,\n    \"reference_K_gapped\": %.4f,\n"
+                   "    \"reference_lambda_gapped\": %.4f,\n",
+                opt->ung_K, opt->ung_lambda, opt->gap_K, opt->gap_lambda);
+            put(o, "    \"query_length\": %d,\n    \"database_size\": %zu,\n    \"effective_query_length\": %d,\n"
+                   "    \"effective_database_size\": %zu,\n    \"effective_search_space\": %zu\n  }\n}}\n",
+                in->query_len, (size_t)in->db_size, effq, (size_t)effdb, (size_t)sspace);
+        }
+    } catch (const std::bad_alloc&) {
+        return CBP_ERR_MEM;
     }
-    return CBP_OK;
+    return finish(o, out, cap, len);
 }

@@ -1,11 +1,12 @@
-"""A query's search results as the reference's plain-text output
-(cbp_format_plain), and the search space behind its summary
-(cbp_search_space).
+"""A query's search results as the reference's output files: plain text
+(cbp_format_plain) and JSON (cbp_format_json), and the search space behind
+their summaries (cbp_search_space).
 
 ``format_plain(result, options, query=..., db=...)`` takes a result dict of
 ``CuBatchProcessing.ProcessScoreMatrix`` / ``ProcessBatch`` / ``SearchDB``
 and returns the text of the reference's per-query output file
-(cualn/AlnWriterPlain.cpp:19-121).
+(cualn/AlnWriterPlain.cpp:19-121); ``format_json`` takes the same
+arguments and returns the JSON file (cualn/AlnWriterJSON.cpp:20-143).
 """
 import ctypes as C
 import numpy as np
@@ -32,8 +33,9 @@ def _L():
     global _lib
     if _lib is None:
         L = C.CDLL(LIB_HIP)
-        L.cbp_format_plain.argtypes = [C.POINTER(Options), C.POINTER(PlainInfo), C.POINTER(Results),
-                                       C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
+        for f in (L.cbp_format_plain, L.cbp_format_json):
+            f.argtypes = [C.POINTER(Options), C.POINTER(PlainInfo), C.POINTER(Results),
+                          C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]
         L.cbp_search_space.argtypes = [C.POINTER(Options), C.c_uint64, C.c_uint64, C.c_uint64,
                                        C.POINTER(C.c_uint64), C.POINTER(C.c_int32)]
         _lib = L
@@ -57,13 +59,22 @@ def _strarr(items):
     return arr
 
 
-def format_plain(result, options, query_desc, query_len, query_eno, db_names, db_size, db_entries,
-                 pro_desc, pro_len, pro_eno, query_sss=None, pro_sss=None, pro_dist=None,
-                 deltalen=None, nseqs=None, progname=None, progversion=None, references=None,
-                 nohits=700, noalns=700, dsclen=1024, dscwidth=80, alnwidth=60, show=1):
-    """the output text of one query (see the module doc); pro_* are indexed by
-    the records' profile index; deltalen defaults to the length adjustment
-    of (query_len, db_size, nseqs or db_entries)"""
+def format_plain(result, options, *args, **kw):
+    """the plain-text output of one query (see the module doc)"""
+    return _format("cbp_format_plain", result, options, *args, **kw)
+
+
+def format_json(result, options, *args, **kw):
+    """the JSON output of one query (see the module doc)"""
+    return _format("cbp_format_json", result, options, *args, **kw)
+
+
+def _format(fn, result, options, query_desc, query_len, query_eno, db_names, db_size, db_entries,
+            pro_desc, pro_len, pro_eno, query_sss=None, pro_sss=None, pro_dist=None,
+            deltalen=None, nseqs=None, progname=None, progversion=None, references=None,
+            nohits=700, noalns=700, dsclen=1024, dscwidth=80, alnwidth=60, show=1):
+    """pro_* are indexed by the records' profile index; deltalen defaults to
+    the length adjustment of (query_len, db_size, nseqs or db_entries)"""
     n = int(result["npassed"])
     nm = int(result["nmap"])
     a = np.ascontiguousarray(result["alndata"], np.float32).reshape(n, _ffi.NFIELDS) if n else np.zeros((1, _ffi.NFIELDS), np.float32)
@@ -107,11 +118,11 @@ def format_plain(result, options, query_desc, query_len, query_eno, db_names, db
     info.nohits, info.noalns = nohits, noalns
     info.dsclen, info.dscwidth, info.alnwidth, info.show = dsclen, dscwidth, alnwidth, show
     o = _ffi.options_struct(options)
-    L = _L()
+    f = getattr(_L(), fn)
     size = C.c_size_t()
-    if L.cbp_format_plain(C.byref(o), C.byref(info), C.byref(r), None, 0, C.byref(size)) != 0:
-        raise ValueError("cbp_format_plain: invalid arguments")
+    if f(C.byref(o), C.byref(info), C.byref(r), None, 0, C.byref(size)) != 0:
+        raise ValueError(fn + ": invalid arguments")
     buf = C.create_string_buffer(size.value + 1)
-    if L.cbp_format_plain(C.byref(o), C.byref(info), C.byref(r), buf, size.value + 1, C.byref(size)) != 0:
-        raise ValueError("cbp_format_plain: invalid arguments")
+    if f(C.byref(o), C.byref(info), C.byref(r), buf, size.value + 1, C.byref(size)) != 0:
+        raise ValueError(fn + ": invalid arguments")
     return buf.raw[:size.value].decode("latin-1")

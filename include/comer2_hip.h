@@ -298,6 +298,18 @@ typedef struct {
 int  cbp_format_plain(const cbp_options* opt, const cbp_plain_info* info, const cbp_results* res,
                       char* out, size_t cap, size_t* len);
 
+/* the same results as the reference's JSON output file (host only):
+ * {"comer_search": {program, version, references, query, database,
+ * message, search_summary, search_hits, search_statistics}}
+ * (AlnWriter::WriteResultsJSON, cualn/AlnWriterJSON.cpp:20-143, with the
+ * records of CuBatchProcessingFinalizer::CompressResultsJSON,
+ * cupro/CuBatchProcessingFinalizerJSON.cpp:35-382); descriptions escape
+ * '"', '\\' and '/' (FormatDescriptionJSON, liblib/fmtdescription.h:96-126).
+ * Takes the fields of cbp_plain_info that the JSON file uses (dscwidth,
+ * alnwidth and show are not). */
+int  cbp_format_json(const cbp_options* opt, const cbp_plain_info* info, const cbp_results* res,
+                     char* out, size_t cap, size_t* len);
+
 /* effective search space and length adjustment of a query of qlen
  * positions against dblen positions in nseqs profiles
  * (AbstractScoreMatrix::ComputeLengthAdjustment,

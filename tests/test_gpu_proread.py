@@ -67,6 +67,21 @@ def _check_output(res, q, qp, d, opts):
         qs = res["alns"][j][0]
         frag = "".join(l[13:-6].rstrip() for l in txt.split("\n") if l.startswith("Query: "))
         assert qs[:60] in frag
+    # the JSON file of the same results parses, and its records carry the
+    # drawn strings in the same order
+    import json
+    from comer2_amd.output import format_json
+    js = format_json(res, o, q.desc(qp), ql, float(q.view.eno[qp]), ["db.pro"], d.npos, d.nprof,
+                     [d.desc(i) for i in range(d.nprof)], d.lengths(), [d.view.eno[i] for i in range(d.nprof)],
+                     query_sss=qsss, pro_sss=sss, pro_dist=d.dists())
+    doc = json.loads(js)["comer_search"]
+    recs = [h["hit_record"] for h in doc["search_hits"]]
+    assert len(recs) == len(rows) == len(doc["search_summary"])
+    for j, rec in enumerate(recs):
+        a = rec["alignment"]
+        assert rec["target_description"] == descs[j]
+        assert (a["query_aln"], a["middle"], a["target_aln"]) == tuple(res["alns"][j])
+        assert a["query_secstr_available"] == 1 and len(a["query_secstr"]) == len(a["query_aln"])
 
 
 def test_binary_database_search_equals_text(tmp_path):
