@@ -426,7 +426,10 @@ extern "C" int cbp_format_json(const cbp_options* opt, const cbp_plain_info* in,
                 o += "  \"references\": [\n";
                 int totlen = 0;
                 bool bwrt = true;
-                for (int i = 0; !i || (bwrt && refs[i]); i++) {
+                /* one line at least (empty when there are no references);
+                 * the list ends at its first null entry, which is never read
+                 * past */
+                for (int i = 0; !i || (bwrt && refs[i - 1] && refs[i]); i++) {
                     const char* r = refs[i];
                     totlen += r ? (int)std::strlen(r) : 0;
                     bwrt = totlen < 1024 - 60;

@@ -184,3 +184,12 @@ def test_json_database_names_limit():
     assert '      "n40",\n      "..."\n    ],\n' in got and '"n41"' not in got
     # the message needs 360 more: gone once the names took more than 384
     assert '"message"' not in got
+
+
+def test_json_without_references():
+    # no reference list: one empty entry, and nothing is read past the list
+    opts = dict(tables.default_options(), ssswgt=0.0)
+    got = _fmt(opts, references=None)
+    want = _expected(opts).replace('    "Ref one.",\n    "Ref two words."\n', '    ""\n')
+    assert got == want, _diff(got, want)
+    assert json.loads(got)["comer_search"]["references"] == [""]
