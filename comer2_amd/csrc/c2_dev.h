@@ -7,6 +7,18 @@
 #include <stdint.h>
 #include "c2_fmath.h"
 
+/* raise a kernel's dynamic LDS limit once per process; the block-scope
+ * static is initialised exactly once even when several host threads (one
+ * handle each) launch the kernel for the first time together */
+#define C2_LDS_LIMIT_ONCE(fn, bytes)                                                              \
+    do {                                                                                          \
+        static const bool c2_once_ = ((void)hipFuncSetAttribute((const void*)(fn),                \
+                                                                hipFuncAttributeMaxDynamicSharedMemorySize, \
+                                                                (int)(bytes)),                    \
+                                      true);                                                      \
+        (void)c2_once_;                                                                           \
+    } while (0)
+
 /* the five DP states of the reference (cudp/CuBatchDP_com.h:77-84) */
 struct S5 { float mm, mi, im, dn, nd; };
 

@@ -438,11 +438,7 @@ hipError_t c2_dl_pass(const FinRec* fin, const int32_t* cand, const int32_t* len
 hipError_t c2_dl_order_select(const double* statout, const DlBufs& b, const DlSelArgs& sa, hipStream_t st)
 {
     const size_t lds = sizeof(unsigned long long) * (size_t)DL_SORT;
-    static bool attr = false;
-    if (!attr) {
-        (void)hipFuncSetAttribute((const void*)k_dl_order_select, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
-        attr = true;
-    }
+    C2_LDS_LIMIT_ONCE(k_dl_order_select, lds);
     hipLaunchKernelGGL(k_dl_order_select, dim3(1), dim3(BT), lds, st, statout, b, sa);
     return hipGetLastError();
 }

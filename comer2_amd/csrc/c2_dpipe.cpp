@@ -35,16 +35,20 @@ struct DlCaps {
     int mR, mns, mnw;
 };
 
-DlCaps dl_caps(cbp_handle* h, const std::vector<int>& lq)
+DlCaps dl_caps(cbp_handle* h, const std::vector<int>& lq, const std::vector<int>& nstripes)
 {
     DlCaps k{};
     const int np = h->dchunk.nprof, maxlen = h->chunk_maxlen > 0 ? h->chunk_maxlen : 1;
-    int Lmax = 1;
+    int Lmax = 1, smax = 1;
     for (int L : lq) Lmax = std::max(Lmax, L);
+    for (int s : nstripes) smax = std::max(smax, s);
     k.capC = std::max(1, std::min(h->dl_capC, np));
     k.capP = std::max(1, std::min(h->dl_capP, k.capC));
     k.capM = h->opt.max_map_pairs > 0 ? std::min(h->opt.max_map_pairs, k.capP) : k.capP;
-    k.capW = h->dl_capW;
+    /* no more code words than every profile of the chunk passing would take
+     * (stripes x (len + 63) x 64 each), so a small chunk holds a small pool */
+    const int64_t allw = (int64_t)smax * ((int64_t)h->dchunk.npos + 63 * (int64_t)np) * 64;
+    k.capW = std::max<int64_t>(64, std::min<int64_t>(h->dl_capW, allw));
     c2_map_geometry(Lmax, &k.mR, &k.mns, &k.mnw);
     const int64_t chunks = (maxlen + C2_MAP_SCORE_COLS - 1) / C2_MAP_SCORE_COLS;
     k.capItems = (int64_t)k.capM * 8 * chunks;
@@ -312,7 +316,7 @@ int c2_run_queries_dev(cbp_handle* h, const int32_t* qidx, int32_t nq, const flo
     while ((int)h->dlq.size() < nq) h->dlq.emplace_back(new DlQuery());
     const int nslots = nq < C2_NSTREAMS ? nq : C2_NSTREAMS;
     h->qtick++;
-    std::vector<int> lq(nq);
+    std::vector<int> lq(nq), nstr(nq);
     for (int i = 0; i < nq; i++) {
         QWork& w = *h->work[i];
         w.qidx = qidx[i];
@@ -323,6 +327,7 @@ int c2_run_queries_dev(cbp_handle* h, const int32_t* qidx, int32_t nq, const flo
         w.dq = c2_get_query(h, qidx[i], &rc);
         if (!w.dq) return rc;
         lq[i] = w.dq->hp.Lq;
+        nstr[i] = w.dq->hp.nstripes;
     }
     DCHK(hipStreamSynchronize(h->stream));
     const bool queue = h->use_queue && nq >= 2;
@@ -339,7 +344,7 @@ int c2_run_queries_dev(cbp_handle* h, const int32_t* qidx, int32_t nq, const flo
     }
     /* capacities, buffers and the per-query constants */
     DlPlan P;
-    P.k = dl_caps(h, lq);
+    P.k = dl_caps(h, lq, nstr);
     P.split = (nq == 1);
     P.sp.resize(nq); P.grid.resize(nq); P.lds.resize(nq);
     const cbp_profiles& ch = h->chunk.v;
@@ -367,12 +372,12 @@ int c2_run_queries_dev(cbp_handle* h, const int32_t* qidx, int32_t nq, const flo
         sp.gap_lambda = h->opt.gap_lambda; sp.gap_K = h->opt.gap_K;
     }
     /* a buffer that grows moves: graphs that hold the old addresses retire */
-    auto addrs = [&]() {
-        std::vector<void*> v;
+    auto bufs = [&]() {
+        std::vector<DBuf*> v;
         for (int i = 0; i < nq; i++) {
             DlQuery& q = *h->dlq[i];
             for (DBuf* b : {&q.cnt, &q.aln, &q.epa, &q.order, &q.sel, &q.res, &q.p1save, &q.malns, &q.dargs, &q.tiles, &q.ntiles})
-                v.push_back(b->p);
+                v.push_back(b);
         }
         for (int s = 0; s < nslots; s++) {
             DlSlot& sl = h->dls[s];
@@ -380,13 +385,29 @@ int c2_run_queries_dev(cbp_handle* h, const int32_t* qidx, int32_t nq, const flo
                             &sl.pass_pro, &sl.passfin, &sl.statin, &sl.counts, &sl.statout, &sl.mpairs, &sl.morder,
                             &sl.sitems, &sl.apro, &sl.acand, &sl.mF, &sl.mB, &sl.mSd, &sl.mbt, &sl.mhand,
                             &sl.mstat, &sl.malntmp})
-                v.push_back(b->p);
+                v.push_back(b);
         }
         return v;
     };
+    auto addrs = [&]() {
+        std::vector<void*> v;
+        for (DBuf* b : bufs()) v.push_back(b->p);
+        return v;
+    };
     const std::vector<void*> a0 = addrs();
-    for (int i = 0; i < nq; i++) if ((rc = dl_alloc_query(h, *h->dlq[i], P.k))) return rc;
-    for (int s = 0; s < nslots; s++) if ((rc = dl_alloc_slot(h, h->dls[s], P.k, botw, P.split))) return rc;
+    rc = CBP_OK;
+    for (int i = 0; i < nq && !rc; i++) rc = dl_alloc_query(h, *h->dlq[i], P.k);
+    for (int s = 0; s < nslots && !rc; s++) rc = dl_alloc_slot(h, h->dls[s], P.k, botw, P.split);
+    if (rc == CBP_ERR_MEM) {
+        /* the lists' fixed pools do not fit the memory budget: give them back
+         * and run the host-decided sequence, which sizes its buffers to the
+         * call */
+        c2_drop_graphs(h);
+        for (DBuf* b : bufs()) b->release();
+        h->last_dl_fallbacks = nq;
+        return c2_run_queries(h, qidx, nq, scorethld, logevthld, outs);
+    }
+    if (rc) return rc;
     if (addrs() != a0) c2_drop_graphs(h);
     /* the backtrack pass's arguments (the device writes their segment count) */
     for (int i = 0; i < nq; i++) {

@@ -601,12 +601,7 @@ static size_t phase1_cv_lds_bytes(int R, int NW)
 template <int RR, int NWW, bool BT>
 static hipError_t launch_cv(const P1Args& a, int grid, size_t lds, hipStream_t stream)
 {
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void*)k_phase1<RR, NWW, BT, true>,
-                                  hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
-    }
+    C2_LDS_LIMIT_ONCE((k_phase1<RR, NWW, BT, true>), 160 * 1024);
     hipLaunchKernelGGL((k_phase1<RR, NWW, BT, true>), dim3(grid), dim3(64 * NWW), lds, stream, a);
     return hipGetLastError();
 }
@@ -619,13 +614,12 @@ static hipError_t launch_cv(const P1Args& a, int grid, size_t lds, hipStream_t s
 template <int NWW, bool BT>
 static hipError_t launch_nw(const P1Args& a, int R, int grid, size_t lds, hipStream_t stream)
 {
-    static bool attr_set[6] = {false, false, false, false, false, false};
     switch (R) {
-        case 1: if (!attr_set[1]) { (void)hipFuncSetAttribute((const void*)k_phase1<1, NWW, BT>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024); attr_set[1] = true; } break;
-        case 2: if (!attr_set[2]) { (void)hipFuncSetAttribute((const void*)k_phase1<2, NWW, BT>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024); attr_set[2] = true; } break;
-        case 3: if (!attr_set[3]) { (void)hipFuncSetAttribute((const void*)k_phase1<3, NWW, BT>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024); attr_set[3] = true; } break;
-        case 4: if (!attr_set[4]) { (void)hipFuncSetAttribute((const void*)k_phase1<4, NWW, BT>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024); attr_set[4] = true; } break;
-        case 5: if (!attr_set[5]) { (void)hipFuncSetAttribute((const void*)k_phase1<5, NWW, BT>, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024); attr_set[5] = true; } break;
+        case 1: C2_LDS_LIMIT_ONCE((k_phase1<1, NWW, BT>), 160 * 1024); break;
+        case 2: C2_LDS_LIMIT_ONCE((k_phase1<2, NWW, BT>), 160 * 1024); break;
+        case 3: C2_LDS_LIMIT_ONCE((k_phase1<3, NWW, BT>), 160 * 1024); break;
+        case 4: C2_LDS_LIMIT_ONCE((k_phase1<4, NWW, BT>), 160 * 1024); break;
+        case 5: C2_LDS_LIMIT_ONCE((k_phase1<5, NWW, BT>), 160 * 1024); break;
         default: return hipErrorInvalidValue;
     }
     switch (R) {

@@ -3,8 +3,10 @@ FP64 oracle, FP64 central finite differences, the MAP posteriors of the plain
 path, and through the torch.autograd binding.
 
 Tolerances: log Z within 1e-5 relative; dS and the posteriors within 1e-4
-absolute (they lie in [0, 1]); dtheta within 1e-3 relative; gradients vs FP64
-central differences (step 1e-3) within 1e-3.
+absolute (they lie in [0, 1]) and, for pairs of 50-700 x 30-400 (the
+moderate-length differentiation workload), the posteriors above 1e-2 within
+1e-4 relative; dtheta within 1e-3 relative; gradients vs FP64 central
+differences (step 1e-3) within 1e-3.
 """
 import numpy as np
 import pytest
@@ -55,6 +57,9 @@ def test_soft_dp_matches_fp64_oracle(ckpt):
         assert abs(logZ[k] - z) <= 1e-5 * abs(z), ctx
         assert np.max(np.abs(dS[o:o + L1 * L2] - eS.ravel())) < 1e-4, ctx
         assert np.max(np.abs(post[o:o + L1 * L2] - epost.ravel())) < 1e-4, ctx
+        big = epost.ravel() > 1e-2
+        rel = np.abs(post[o:o + L1 * L2][big] - epost.ravel()[big]) / epost.ravel()[big]
+        assert big.any() and np.max(rel) < 1e-4, (ctx, float(np.max(rel)))
         assert np.all(np.abs(dth[k] - eth) <= 1e-3 * np.maximum(1.0, np.abs(eth))), (ctx, dth[k], eth)
 
 
