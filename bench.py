@@ -322,6 +322,34 @@ def softdp_bench(reps):
     return out
 
 
+def phase1_variants(cbp, q, d):
+    """phase-1 launches of the C1 query (and of a 256-row query: one stripe
+    at 4 rows per lane, where neither sweep spills) with the 20-term score
+    sums on the vector ALUs and on the matrix cores; keys must be identical"""
+    import numpy as np
+    from comer2_amd._ffi import SynthDB
+    out = {"metric": "phase-1 sweep time, vector vs matrix-core score sums", "unit": "ms", "legs": []}
+    for qs in (q, SynthDB(QUERY_SEED + 1, 1, 256, 0.0, 256, 256)):
+        cbp.TransferQueries(qs)
+        L = int(qs.lengths()[0])
+        keys = {}
+        for on in (False, True):
+            cbp.set_mfma(on)
+            keys[on] = cbp.phase1_keys(0)
+            ms = []
+            for _ in range(3):
+                cbp.phase1_keys(0)
+                m, n = cbp.last_kernel_ms()
+                ms.append(m / max(n, 1))
+            leg = dict(query_len=L, matrix_cores=on, kernel_ms=round(min(ms), 3),
+                       gcups=round(L * d.npos / (min(ms) * 1e-3) / 1e9, 2))
+            out["legs"].append(leg)
+            log("bench: phase-1 L=%d %s: %.2f ms" % (L, "matrix cores" if on else "vector ALUs", min(ms)))
+        out["legs"][-1]["keys_identical"] = bool(np.array_equal(keys[False], keys[True]))
+    cbp.set_mfma(False)
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -335,6 +363,12 @@ def main():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--softdp-reps", type=int, default=3)
     ap.add_argument("--no-stream", action="store_true")
+    ap.add_argument("--map-rows", type=int, default=-1,
+                    help="rows per lane of the shorter pairs' MAP group (0-5; default: the library's)")
+    ap.add_argument("--phase1-variants", action="store_true",
+                    help="only time the phase-1 sweep of the C1 query with its score sums on the vector "
+                         "ALUs and on the matrix cores (and a 256-row query, one stripe at 4 rows per lane), "
+                         "keys compared; one JSON line")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU; the search shards over DB profiles "
@@ -360,6 +394,8 @@ def main():
     log("bench: generated %d profiles, %d positions in %.1f s" % (d.nprof, d.npos, time.perf_counter() - t))
 
     cbp = CuBatchProcessing(0, 0, 0, options=dict(max_map_pairs=MAP_CAP))
+    if args.map_rows >= 0:
+        cbp.set_map_rows(args.map_rows)
     t = time.perf_counter()
     cbp.TransferQueries(q)
     cbp.TransferCPMData(d)
@@ -367,6 +403,10 @@ def main():
     thld, logev = cbp.score_threshold(QUERY_LEN, d.npos)
     log("bench: uploaded in %.1f s; score threshold %.3f, log E threshold %.3f"
         % (time.perf_counter() - t, thld, logev))
+
+    if args.phase1_variants:
+        phase1_variants(cbp, q, d)
+        return
 
     for i in range(args.warmup):
         res = cbp.ProcessScoreMatrix(0, scorethld=thld, logevthld=logev)

@@ -8,6 +8,24 @@ LIB_HIP = os.path.join(_HERE, "libcomer2_hip.so")
 LIB_SYNTH = os.path.join(_HERE, "libc2synth.so")
 
 
+def load_hip():
+    """libcomer2_hip.so, opened after torch when torch is importable.
+    PyTorch-ROCm ships its own HIP runtime under the soname of
+    /opt/rocm's (libamdhip64.so.7): with torch loaded first the library binds
+    to that runtime, so the process has one HIP runtime.  Opened before
+    torch, it binds to /opt/rocm's and a later torch brings a second one, and
+    then calls of either see spurious errors (torch finds no GPU, kernel
+    launches report an unknown error).  Every loader of the library goes
+    through here."""
+    if not os.path.exists(LIB_HIP):
+        raise OSError("libcomer2_hip.so is not built (run comer2_amd.build)")
+    try:
+        import torch  # noqa: F401
+    except ImportError:
+        pass
+    return C.CDLL(LIB_HIP)
+
+
 class Profiles(C.Structure):
     _fields_ = [("nprof", C.c_int32), ("npos", C.c_int64),
                 ("bkg", C.POINTER(C.c_float)), ("eno", C.POINTER(C.c_float)),

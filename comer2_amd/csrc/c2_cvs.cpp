@@ -206,6 +206,7 @@ void c2_cv_constants(float* kappa_inv, float* nu_pow, float* cterm)
 extern "C" int cbp_cvs2s_read(const char* path, float* keys, float* values, int32_t cap,
                               int32_t* card, int32_t* shift)
 {
+    C2ApiScope c2_scope_;
     if (!path || !card || !shift) return CBP_ERR_ARG;
     std::vector<float> k, s;
     std::string err;

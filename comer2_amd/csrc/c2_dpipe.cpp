@@ -58,6 +58,11 @@ DlCaps dl_caps(cbp_handle* h, const std::vector<int>& lq, const std::vector<int>
         c2_map_geometry(L, &R, &ns, &nw);
         cells = std::max(cells, c2_map_cells(R, ns, maxlen));
         hand = std::max(hand, (c2_map_hand(ns, maxlen) + 63) & ~(int64_t)63);
+        if (h->map_rmax2 > 0) {
+            c2_map_geometry_short(L, h->map_rmax2, &R, &ns, &nw);
+            cells = std::max(cells, c2_map_cells(R, ns, maxlen));
+            hand = std::max(hand, (c2_map_hand(ns, maxlen) + 63) & ~(int64_t)63);
+        }
     }
     k.capCells = std::min<int64_t>(h->dl_capCells, (int64_t)k.capM * cells);
     k.capHand = std::min<int64_t>((int64_t)1 << 26, (int64_t)k.capM * hand);
@@ -199,12 +204,16 @@ int dl_chain(cbp_handle* h, int i, const DlPlan& P)
                          k.capP, sl.statout.as<double>(), st, q.cnt.as<int32_t>() + DL_NPASS));
     int mR = 1, mns = 1, mnw = 1;
     c2_map_geometry(hp.Lq, &mR, &mns, &mnw);
+    /* the shorter pairs of a split call: fewer, busier waves per pair */
+    int mR2 = mR, mns2 = mns, mnw2 = mnw;
+    if (P.split && h->map_rmax2 > 0) c2_map_geometry_short(hp.Lq, h->map_rmax2, &mR2, &mns2, &mnw2);
     DlSelArgs sa{};
     sa.len = d.len.as<int32_t>();
     sa.logevthld = w.logevthld;
     sa.cap = h->opt.max_map_pairs;
     sa.mapaln = h->opt.mapaln;
     sa.mR = mR; sa.mns = mns;
+    sa.mR2 = mR2; sa.mns2 = mns2;
     sa.modinuse = (h->opt.ssswgt > 0.0f || h->opt.cvswgt > 0.0f) ? 1 : 0;
     sa.split = P.split ? 1 : 0;
     DCHK(c2_dl_order_select(sl.statout.as<double>(), b, sa, st));
@@ -231,6 +240,7 @@ int dl_chain(cbp_handle* h, int i, const DlPlan& P)
             DCHK(hipStreamWaitEvent(sA, h->gfork, 0));
             MapArgs mA = ma, mB = ma;
             mA.group = 0; mB.group = 1;
+            mB.R = mR2; mB.ns = mns2; mB.nw = mnw2;
             DCHK(c2_launch_map(sc, mA, k.capM, sA, h->xstream[1], h->mfork[0], h->mjoin[0]));
             DCHK(c2_launch_map(sc, mB, k.capM, st, h->xstream[2], h->mfork[1], h->mjoin[1]));
             DCHK(hipEventRecord(h->gjoin[0], sA));
@@ -297,6 +307,7 @@ int dl_build_graph(cbp_handle* h, int nq, int nslots, bool queue, const DlPlan& 
 int c2_run_queries_dev(cbp_handle* h, const int32_t* qidx, int32_t nq, const float* scorethld,
                        const float* logevthld, cbp_results* outs)
 {
+    (void)hipGetLastError();   /* an earlier call's unreported status is not this call's */
     if (!h->have_opt || !h->have_tables || !h->have_queries || !h->have_chunk)
         return c2_fail(h, CBP_ERR_STATE, "options, tables, queries and chunk must be uploaded first");
     if (!h->dnnw.p) return c2_fail(h, CBP_ERR_STATE, "SS18 parameters not uploaded");

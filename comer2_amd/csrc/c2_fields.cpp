@@ -43,6 +43,7 @@ size_t fbytes(const void* const* beg, const void* const* end, int f)
 
 extern "C" int cbp_fields_count(const void* const* beg, const void* const* end, int32_t* nprof, int64_t* npos)
 {
+    C2ApiScope c2_scope_;
     if (!beg || !end || !nprof || !npos) return CBP_ERR_ARG;
     const size_t np = fbytes(beg, end, F_LEN) / sizeof(int32_t);
     const size_t ns = fbytes(beg, end, F_AA);
@@ -67,6 +68,7 @@ extern "C" int cbp_fields_pack(const void* const* beg, const void* const* end,
                                float* bkg, float* eno, int32_t* len, uint32_t* dist,
                                float* trgfrq, float* ssprb, char* aa, char* sss, float* trans)
 {
+    C2ApiScope c2_scope_;
     int32_t np32 = 0;
     int64_t ns64 = 0;
     int rc = cbp_fields_count(beg, end, &np32, &ns64);
@@ -97,6 +99,7 @@ extern "C" int cbp_fields_pack(const void* const* beg, const void* const* end,
 extern "C" int cbp_fields_pack_cv(const void* const* beg, const void* const* end,
                                   float* cv, float* cvprior, float* cvnorm2)
 {
+    C2ApiScope c2_scope_;
     int32_t np32 = 0;
     int64_t ns64 = 0;
     int rc = cbp_fields_count(beg, end, &np32, &ns64);
@@ -148,6 +151,7 @@ struct FieldSet {
 
 extern "C" int cbp_upload_chunk_fields(cbp_handle* h, const void* const* beg, const void* const* end)
 {
+    C2ApiScope c2_scope_;
     if (!h) return CBP_ERR_ARG;
     try {
         FieldSet fs;
@@ -163,6 +167,7 @@ extern "C" int cbp_upload_chunk_fields(cbp_handle* h, const void* const* beg, co
  * advanced to the query) and its length */
 extern "C" int cbp_add_query_fields(cbp_handle* h, const void* const* beg, uint64_t nqyposs, int32_t* qidx)
 {
+    C2ApiScope c2_scope_;
     if (!h || !beg || !qidx || nqyposs < 1 || nqyposs > 65535) return CBP_ERR_ARG;
     const char* end[CBP_PM_NFIELDS];
     for (int f = 0; f < CBP_PM_NFIELDS; f++) {
@@ -190,6 +195,7 @@ extern "C" int cbp_add_query_fields(cbp_handle* h, const void* const* beg, uint6
 extern "C" int cbp_upload_db_fields(cbp_handle* h, const void* const* beg1, const void* const* end1,
                                     const void* const* beg2, const void* const* end2)
 {
+    C2ApiScope c2_scope_;
     if (!h) return CBP_ERR_ARG;
     if (!beg1 || !end1) return cbp_upload_chunk_fields(h, beg2, end2);
     if (!beg2 || !end2) return cbp_upload_chunk_fields(h, beg1, end1);

@@ -44,6 +44,7 @@ struct P1Args {
     const int32_t* seg_pb; /* [nseg] first profile of each segment */
     const int32_t* seg_pe; /* [nseg] one past the last profile */
     int32_t nseg;
+    int32_t mfma;         /* the DB sweep forms the 20-term sums on the matrix cores */
     /* outputs */
     unsigned long long* keys;   /* [nprof] */
     float* bot;                 /* [5][npos] stripe hand-off (nstripes > 1) */
@@ -185,6 +186,7 @@ struct DlSelArgs {
     const int32_t* len;                             /* chunk profile lengths */
     float logevthld;
     int32_t cap, mapaln, mR, mns, modinuse, split;
+    int32_t mR2, mns2;     /* geometry of the shorter pairs' group (split calls) */
 };
 hipError_t c2_dl_candidates(const unsigned long long* keys, int32_t np, float thld, const DlBufs& b,
                             const int32_t* len, int32_t nstripes, P1Args* dargs, P1Tile* tiles, int32_t qslot,
@@ -197,6 +199,9 @@ hipError_t c2_dl_results(const DlBufs& b, hipStream_t st);
 
 /* MAP geometry for a query: rows per lane and waves per pair */
 void c2_map_geometry(int Lq, int* R, int* ns, int* nw);
+/* the geometry of the shorter pairs' group: at most rmax rows per lane, as
+ * few stripes as that allows (fewer waves per pair, each busier) */
+void c2_map_geometry_short(int Lq, int rmax, int* R, int* ns, int* nw);
 /* score pass, then the forward pass and log Z on stream while the backward
  * pass runs on bstream (the two read only S'), then the MAP DP and the
  * backtrace on stream; bstream == nullptr runs every pass on stream; fork and
@@ -233,8 +238,9 @@ hipError_t c2_launch_stats(const StatParams& sp, const float* nnw /*device [93]*
                            double* out /* [n][10] */, hipStream_t stream, const int32_t* n_dev = nullptr);
 hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream_t stream);
 /* persistent workgroups that fill the device once for k_phase1q<R,NW,btck> with lds bytes */
-int c2_phase1q_grid(int R, int NW, bool btck, size_t lds);
-hipError_t c2_launch_phase1q(const P1QueueArgs& q, int R, int NW, bool btck, int grid, size_t lds, hipStream_t stream);
+int c2_phase1q_grid(int R, int NW, bool btck, size_t lds, bool mf = false);
+hipError_t c2_launch_phase1q(const P1QueueArgs& q, int R, int NW, bool btck, int grid, size_t lds, hipStream_t stream,
+                             bool mf = false);
 
 /* soft DP: one pair of (query, chunk profile) and its workspace offsets (floats) */
 struct SdpPair {

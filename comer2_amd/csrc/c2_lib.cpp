@@ -57,6 +57,9 @@ int c2_dalloc(cbp_handle* h, DBuf& b, size_t n) {
     if (h->budget && h->used + n > h->budget)
         return fail(h, CBP_ERR_MEM, "device memory budget exceeded");
     HIPCHK(hipMalloc(&b.p, n));
+    /* a status left pending by a successful allocation (seen with two HIP
+     * runtimes in the process) must not reach the next launch check */
+    (void)hipGetLastError();
     b.n = n; b.acct = &h->used; h->used += n;
     return CBP_OK;
 }
@@ -91,6 +94,7 @@ extern "C" {
 
 int cbp_create(int device, size_t mem_budget, int area, cbp_handle** out)
 {
+    C2ApiScope c2_scope_;
     (void)area;
     if (!out) return CBP_ERR_ARG;
     *out = nullptr;
@@ -114,6 +118,7 @@ int cbp_create(int device, size_t mem_budget, int area, cbp_handle** out)
 
 int cbp_destroy(cbp_handle* h)
 {
+    C2ApiScope c2_scope_;
     if (!h) return CBP_ERR_ARG;
     (void)hipSetDevice(h->device);
     (void)hipStreamSynchronize(h->stream);
@@ -143,6 +148,9 @@ int cbp_destroy(cbp_handle* h)
     if (h->ev1) (void)hipEventDestroy(h->ev1);
     if (h->stream) (void)hipStreamDestroy(h->stream);
     delete h;
+    /* the teardown's own failures are not reported; clear the thread's last
+     * error so that another handle's next launch check does not read it */
+    (void)hipGetLastError();
     return CBP_OK;
 }
 
@@ -153,6 +161,7 @@ const char* cbp_last_error(const cbp_handle* h)
 
 int cbp_set_options(cbp_handle* h, const cbp_options* opt)
 {
+    C2ApiScope c2_scope_;
     if (!h || !opt) return CBP_ERR_ARG;
     if (!(opt->ssswgt >= 0.0f) || !(opt->cvswgt >= 0.0f) || !(opt->gap_lambda > 0.0f) || !(opt->gap_K > 0.0f))
         return fail(h, CBP_ERR_ARG, "invalid options");
@@ -171,6 +180,7 @@ int cbp_set_options(cbp_handle* h, const cbp_options* opt)
 
 int cbp_upload_tables(cbp_handle* h, const float* ssss900, const float* nnw, int32_t nnw_count)
 {
+    C2ApiScope c2_scope_;
     if (!h || !ssss900) return CBP_ERR_ARG;
     h->ssss.assign(ssss900, ssss900 + 900);
     if (nnw) {
@@ -188,6 +198,7 @@ int cbp_upload_tables(cbp_handle* h, const float* ssss900, const float* nnw, int
 
 int cbp_upload_queries(cbp_handle* h, const cbp_profiles* q)
 {
+    C2ApiScope c2_scope_;
     if (!h) return CBP_ERR_ARG;
     if (!check_set(q)) return fail(h, CBP_ERR_ARG, "invalid query profile set");
     h->queries.copy_from(q);
@@ -300,6 +311,7 @@ extern "C" {
 
 int cbp_upload_chunk(cbp_handle* h, const cbp_profiles* db)
 {
+    C2ApiScope c2_scope_;
     if (!h) return CBP_ERR_ARG;
     if (!check_set(db)) return fail(h, CBP_ERR_ARG, "invalid database profile set");
     (void)hipSetDevice(h->device);
@@ -342,6 +354,7 @@ int cbp_upload_chunk(cbp_handle* h, const cbp_profiles* db)
 
 int cbp_set_db_details(cbp_handle* h, uint64_t dblen, uint64_t nseqs)
 {
+    C2ApiScope c2_scope_;
     if (!h) return CBP_ERR_ARG;
     h->dblen = dblen; h->nseqs = nseqs;
     return CBP_OK;
@@ -350,6 +363,7 @@ int cbp_set_db_details(cbp_handle* h, uint64_t dblen, uint64_t nseqs)
 int cbp_score_threshold(const cbp_options* o, uint64_t qlen, uint64_t dblen,
                         float* scorethld, float* logevthld)
 {
+    C2ApiScope c2_scope_;
     if (!o || !scorethld || !logevthld) return CBP_ERR_ARG;
     const uint64_t sspace = qlen * dblen;
     float ev = o->evalue;
@@ -449,6 +463,7 @@ int c2_fill_p1args(cbp_handle* h, DevQuery* dq, P1Args* a)
     a->ssswgt = h->opt.ssswgt;
     c2_fill_cvs(h, dq, &a->cvs);
     a->seg_pb = h->dseg_b.as<int32_t>(); a->seg_pe = h->dseg_e.as<int32_t>(); a->nseg = h->nseg;
+    a->mfma = h->use_mfma;
     return CBP_OK;
 }
 
@@ -493,6 +508,7 @@ int c2_ensure_chunk_cv(cbp_handle* h)
 extern "C" int cbp_upload_cvs2s(cbp_handle* h, const float* keys, const float* values, int32_t card,
                                 int32_t shift)
 {
+    C2ApiScope c2_scope_;
     if (!h || !keys || !values) return CBP_ERR_ARG;
     /* the phase-1 kernels keep the map in 128 words of LDS */
     if (card < 1 || card > 128 || shift < 0 || shift >= card)
@@ -513,6 +529,7 @@ extern "C" int cbp_upload_cvs2s(cbp_handle* h, const float* keys, const float* v
 /* phase-1 launch for one query into h->keys; records kernel time */
 int c2_run_phase1(cbp_handle* h, int qidx)
 {
+    (void)hipGetLastError();   /* an earlier call's unreported status is not this call's */
     if (!h->have_opt || !h->have_tables || !h->have_queries || !h->have_chunk)
         return fail(h, CBP_ERR_STATE, "options, tables, queries and chunk must be uploaded first");
     if (qidx < 0 || qidx >= h->queries.v.nprof) return fail(h, CBP_ERR_ARG, "query index out of range");
@@ -542,6 +559,7 @@ int c2_run_phase1(cbp_handle* h, int qidx)
     float ms = 0.0f;
     HIPCHK(hipEventElapsedTime(&ms, h->ev0, h->ev1));
     h->last_ms = ms; h->last_nlaunch = 1;
+    HIPCHK(hipGetLastError());   /* nothing of this call's left unreported */
     return CBP_OK;
 }
 
@@ -549,6 +567,7 @@ extern "C" {
 
 int cbp_phase1_keys(cbp_handle* h, int32_t qidx, uint64_t* keys)
 {
+    C2ApiScope c2_scope_;
     if (!h || !keys) return CBP_ERR_ARG;
     int rc = c2_run_phase1(h, qidx);
     if (rc) return rc;
@@ -558,6 +577,7 @@ int cbp_phase1_keys(cbp_handle* h, int32_t qidx, uint64_t* keys)
 
 int cbp_last_kernel_ms(const cbp_handle* h, double* ms, int32_t* nlaunch)
 {
+    C2ApiScope c2_scope_;
     if (!h || !ms) return CBP_ERR_ARG;
     *ms = h->last_ms;
     if (nlaunch) *nlaunch = h->last_nlaunch;
@@ -567,6 +587,7 @@ int cbp_last_kernel_ms(const cbp_handle* h, double* ms, int32_t* nlaunch)
 /* tuning knobs (autotuner): target segment columns, max rows per lane, waves per group */
 int cbp_set_tuning(cbp_handle* h, int32_t seg_cols, int32_t rmax, int32_t nw)
 {
+    C2ApiScope c2_scope_;
     if (!h || seg_cols < 64 || rmax < 1 || rmax > 5 || !(nw == 1 || nw == 2 || nw == 4 || nw == 8))
         return CBP_ERR_ARG;
     const bool reseg = (seg_cols != h->seg_cols);
@@ -628,6 +649,7 @@ uint64_t c2_length_adjustment(const cbp_options& o, uint64_t m, uint64_t n, uint
 extern "C" int cbp_search_space(const cbp_options* opt, uint64_t qlen, uint64_t dblen, uint64_t nseqs,
                                 uint64_t* sspace, int32_t* deltalen)
 {
+    C2ApiScope c2_scope_;
     if (!opt || !sspace || !deltalen) return CBP_ERR_ARG;
     uint64_t d = 0;
     *sspace = c2_length_adjustment(*opt, qlen, dblen, nseqs, &d);
@@ -655,6 +677,7 @@ static float u2f(uint32_t u) { float f; std::memcpy(&f, &u, 4); return f; }
 
 extern "C" int cbp_score_block(cbp_handle* h, int32_t qidx, int32_t pronr, float* S, float* mod)
 {
+    C2ApiScope c2_scope_;
     if (!h || !S) return CBP_ERR_ARG;
     if (!h->have_opt || !h->have_tables || !h->have_queries || !h->have_chunk)
         return fail(h, CBP_ERR_STATE, "options, tables, queries and chunk must be uploaded first");
@@ -708,6 +731,7 @@ static int sdp_upload_queries(cbp_handle* h)
 extern "C" int cbp_sdp_prepare(cbp_handle* h, const cbp_sdp_pair* pairs, int32_t npairs, int32_t ckpt,
                                int64_t* ws_bytes)
 {
+    C2ApiScope c2_scope_;
     if (!h || !ws_bytes || npairs < 0 || (npairs > 0 && !pairs)) return CBP_ERR_ARG;
     if (!h->have_queries || !h->have_chunk)
         return fail(h, CBP_ERR_STATE, "queries and chunk must be uploaded first");
@@ -795,6 +819,7 @@ static void sdp_fill(cbp_handle* h, SdpArgs* a, const float* S, const float* the
 extern "C" int cbp_sdp_forward(cbp_handle* h, const float* S, const float* theta, float* logZ,
                                void* ws, void* stream)
 {
+    C2ApiScope c2_scope_;
     if (!h || !S || !theta || !logZ || !ws) return CBP_ERR_ARG;
     if (!h->sdp_ready) return fail(h, CBP_ERR_STATE, "soft DP: cbp_sdp_prepare first");
     (void)hipSetDevice(h->device);
@@ -808,6 +833,7 @@ extern "C" int cbp_sdp_forward(cbp_handle* h, const float* S, const float* theta
 extern "C" int cbp_sdp_adjoint(cbp_handle* h, const float* S, const float* theta, const float* logZ,
                                const float* gZ, float* dS, float* dtheta, float* post, void* ws, void* stream)
 {
+    C2ApiScope c2_scope_;
     if (!h || !S || !theta || !logZ || !dS || !dtheta || !ws) return CBP_ERR_ARG;
     if (!h->sdp_ready) return fail(h, CBP_ERR_STATE, "soft DP: cbp_sdp_prepare first");
     (void)hipSetDevice(h->device);
@@ -820,6 +846,7 @@ extern "C" int cbp_sdp_adjoint(cbp_handle* h, const float* S, const float* theta
 
 extern "C" int cbp_sdp_scores(cbp_handle* h, float* S, void* stream)
 {
+    C2ApiScope c2_scope_;
     if (!h || !S) return CBP_ERR_ARG;
     if (!h->sdp_ready || !h->have_tables || !h->have_opt)
         return fail(h, CBP_ERR_STATE, "soft DP: tables, options and cbp_sdp_prepare first");
@@ -843,6 +870,7 @@ extern "C" int cbp_sdp_scores(cbp_handle* h, float* S, void* stream)
  * (segment columns, waves per workgroup) and keeps the fastest */
 extern "C" int cbp_autotune(cbp_handle* h, int32_t qidx, int32_t* seg_cols, int32_t* nw, int32_t* rmax)
 {
+    C2ApiScope c2_scope_;
     if (!h) return CBP_ERR_ARG;
     if (!h->have_opt || !h->have_tables || !h->have_queries || !h->have_chunk)
         return fail(h, CBP_ERR_STATE, "options, tables, queries and chunk must be uploaded first");
@@ -895,6 +923,7 @@ extern "C" int cbp_autotune(cbp_handle* h, int32_t qidx, int32_t* seg_cols, int3
 extern "C" int cbp_chunk_limits(cbp_handle* h, uint64_t qlen, uint64_t* maxpos, uint64_t* maxpros,
                                 uint64_t* maxpos2, uint64_t* maxpros2)
 {
+    C2ApiScope c2_scope_;
     if (!h || !maxpos || !maxpros || !maxpos2 || !maxpros2 || qlen < 1 || qlen > 65535) return CBP_ERR_ARG;
     (void)hipSetDevice(h->device);
     size_t avail = 0;

@@ -13,6 +13,19 @@
 #include "c2_prep.h"
 #include "c2_kernels.h"
 
+/* every ABI call starts and ends with the thread's HIP error status
+ * cleared.  The launch checks read the thread's last error, so a status left
+ * pending by the caller's own HIP work, or by a call that succeeded (seen
+ * when the process held two HIP runtimes: this library opened before
+ * PyTorch's), must not be taken for a launch failure; and a status the
+ * library left behind would be read by the caller's next check */
+struct C2ApiScope {
+    C2ApiScope() { (void)hipGetLastError(); }
+    ~C2ApiScope() { (void)hipGetLastError(); }
+    C2ApiScope(const C2ApiScope&) = delete;
+    C2ApiScope& operator=(const C2ApiScope&) = delete;
+};
+
 /* device buffer; c2_dalloc charges its size to the handle's account
  * (acct), and the release returns it, so the memory budget sees only live
  * buffers */
@@ -244,6 +257,13 @@ struct cbp_handle {
         DBuf dargs, dtiles, dhead, dbot;
     };
     bool use_queue = true;
+    /* the DB sweep forms the 20-term score sums on the matrix cores (the
+     * results are bit-identical either way) */
+    bool use_mfma = false;
+    /* MAP passes of a one-query call: the pairs no longer than half the
+     * longest take at most this many rows per lane (0: the long pairs'
+     * geometry for every pair) */
+    int map_rmax2 = 0;
     QPlan qscan, qcand;
     DBuf qkeys, qwts;                   /* [nq][nprof] maxima, [nq][nprof][20] weights */
     unsigned long long* hqkeys = nullptr; size_t hqkeys_n = 0;

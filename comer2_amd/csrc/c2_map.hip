@@ -810,6 +810,7 @@ template <int R>
 hipError_t launch_r(const ScoreCtx& c, const MapArgs& m0, int npairs, hipStream_t stream,
                     hipStream_t bstream, hipEvent_t fork, hipEvent_t join)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     MapArgs m = m0;
     m.npairs = npairs;
     const int wg = 64 * m.nw;
@@ -854,6 +855,17 @@ void c2_map_geometry(int Lq, int* R, int* ns, int* nw)
     if (r < 1) r = 1;
     if (r > 5) r = 5;
     const int n = (Lq + 64 * r - 1) / (64 * r);
+    *R = r; *ns = n; *nw = n < 8 ? n : 8;
+}
+
+void c2_map_geometry_short(int Lq, int rmax, int* R, int* ns, int* nw)
+{
+    if (rmax < 1) rmax = 1;
+    if (rmax > 5) rmax = 5;
+    int n = (Lq + 64 * rmax - 1) / (64 * rmax);
+    if (n < 1) n = 1;
+    int r = (Lq + 64 * n - 1) / (64 * n);
+    if (r < 1) r = 1;
     *R = r; *ns = n; *nw = n < 8 ? n : 8;
 }
 

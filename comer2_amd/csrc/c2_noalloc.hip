@@ -54,6 +54,7 @@ hipError_t c2_noalloc_init(int device)
 
 extern "C" int cbp_phase1_keys_noalloc(cbp_handle* h, int32_t qidx, uint64_t* keys)
 {
+    C2ApiScope c2_scope_;
     if (!h || !keys) return CBP_ERR_ARG;
     if (!h->have_opt || !h->have_tables || !h->have_queries || !h->have_chunk)
         return c2_fail(h, CBP_ERR_STATE, "options, tables, queries and chunk must be uploaded first");

@@ -55,19 +55,19 @@ constexpr int CV_MAP = 128;                /* LDS words for the CV map */
  * hand-off buffer (stripes after the first) instead of the zero row -1 */
 /* the profile of the column lane loads in the refill of block t0 (loaded
  * half a block ahead, so the refill's loads do not wait on it) */
-template <class PA>
+template <int BK, class PA>
 __device__ __forceinline__ int block_profile(PA& a, int lane, int t0, int64_t c_begin, int64_t c_end)
 {
     const int64_t c = c_begin + t0 + lane;
-    return (lane < BLK && c < c_end) ? a.addrpro[c] : -1;
+    return (lane < BK && c < c_end) ? a.addrpro[c] : -1;
 }
 
-template <int R, bool CV, class PA>
+template <int R, bool CV, int BK, class PA>
 __device__ __forceinline__ void load_block(PA& a, float* ring, int ns, int lane, int t0,
                                            int64_t c_begin, int64_t c_end, bool tops, bool rtop, int p,
                                            const float* bot, int64_t bld, int64_t bc0)
 {
-    if (lane >= BLK) return;
+    if (lane >= BK) return;
     float* rc = ring + ((t0 + lane) % RING) * ns;
     const int64_t c = c_begin + t0 + lane;
     if (c < c_end) {
@@ -155,9 +155,17 @@ typedef const __attribute__((address_space(4))) P1Args CP1Args;
  * botw != nullptr: the stripe hand-off goes to the wave's own scratch rows
  * of bot_ld columns (indexed from the segment's first column) instead of the
  * query's per-column buffer a.bot */
-template <int R, int NW, bool BTCK, bool CV, class PA>
+template <int R, int NW, bool BTCK, bool CV, bool MF, class PA>
 __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw, int64_t bot_ld)
 {
+    static_assert(!(MF && CV), "the matrix-core contraction has no context-vector form");
+    /* refill: BK columns, LEAD steps ahead of the column that starts the
+     * block.  The matrix-core form reads columns up to 3 ahead of lane 0 at
+     * every fourth step, so it refills 16 columns 4 steps ahead (the ring
+     * still holds every column a lane reads: a refill at the end of step t
+     * overwrites columns t-92..t-77, last read at step t-14) */
+    constexpr int BK = MF ? 16 : BLK;
+    constexpr int LEAD = MF ? 4 : 2;
     const int SL = 64 * R;                      /* rows per stripe */
     float* sQT = lds;                           /* [30][SL] */
     float* sLL = sQT + 30 * SL;                 /* [5][SL] */
@@ -249,8 +257,8 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
             }
         }
         wave_lds_sync();
-        int pn = block_profile(a, lane, 0, c_begin, c_end);
-        load_block<R, CV>(a, ring, ns, lane, 0, c_begin, c_end, tops, s > 0, pn, bot, bld, bc0);
+        int pn = block_profile<BK>(a, lane, 0, c_begin, c_end);
+        load_block<R, CV, BK>(a, ring, ns, lane, 0, c_begin, c_end, tops, s > 0, pn, bot, bld, bc0);
         wave_lds_sync();
 
         S5 H[R];
@@ -287,8 +295,10 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
             const __attribute__((address_space(3))) float* rs =
                 (const __attribute__((address_space(3))) float*)(ring + soff);
             asm volatile("" : "+v"(rs));
+            if constexpr (!MF) {
 #pragma unroll
-            for (int k = 0; k < 20; k++) dw[k] = rs[F_DW + k];
+                for (int k = 0; k < 20; k++) dw[k] = rs[F_DW + k];
+            }
 #pragma unroll
             for (int k = 0; k < 7; k++) dt[k] = rs[F_DT + k];
 #pragma unroll
@@ -302,6 +312,68 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
                 for (int i = 0; i < 19; i++) dc[i] = rs[F_CV + i];
                 dpr = rs[F_CVP];
                 dna = rs[F_CVA];
+            }
+        };
+        /* matrix-core form: at every fourth step t = 4s each group of four
+         * lanes (g = lane/4, j = lane%4) forms, on the matrix cores, the
+         * 20-term sums of its rows against four columns with
+         * v_mfma_f32_4x4x1_16b_f32 chained over the 20 terms (one product
+         * added per instruction: the same roundings as the fmaf chain).  Lane
+         * 4g+i supplies column t-4g+i (operand A), lane 4g+j its row's
+         * frequency (operand B); register i of lane 4g+j then holds row j x
+         * column t-4g+i.  Lane 4g+j meets column t+u-4g-j at step t+u, so it
+         * reads register u-j of this tile (u >= j) or 4+u-j of the previous
+         * one; the logarithm is taken on the tile, four cells per row. */
+        float tcur[MF ? R : 1][4], tprv[MF ? R : 1][4];
+        const int jl = lane & 3;
+        if constexpr (MF) {
+            const float z = c2_logf_clamped(0.0f);   /* what an empty (invalid) column scores */
+#pragma unroll
+            for (int k = 0; k < R; k++)
+#pragma unroll
+                for (int i = 0; i < 4; i++) tcur[k][i] = z;
+        }
+        auto tile = [&](int t) {
+            typedef float f4 __attribute__((ext_vector_type(4)));
+#pragma unroll
+            for (int k = 0; k < R; k++)
+#pragma unroll
+                for (int i = 1; i < 4; i++) tprv[k][i] = tcur[k][i];
+            const int ca = t - (lane & ~3) + jl;             /* >= t - 60 */
+            const int slot = (ca + 4 * RING) % RING;
+            const __attribute__((address_space(3))) float* rs =
+                (const __attribute__((address_space(3))) float*)(ring + slot * ns + F_DW);
+            f4 acc[R];
+#pragma unroll
+            for (int k = 0; k < R; k++) acc[k] = f4{0.0f, 0.0f, 0.0f, 0.0f};
+#pragma unroll
+            for (int kk = 0; kk < 20; kk++) {
+                const float av = rs[kk];
+#pragma unroll
+                for (int k = 0; k < R; k++) acc[k] = __builtin_amdgcn_mfma_f32_4x4x1f32(av, q[k][kk], acc[k], 0, 0, 0);
+            }
+            /* the logarithms one column of the tile at a time (R chains) */
+#pragma unroll
+            for (int i = 0; i < 4; i++) {
+                float raw[R], lg[R];
+#pragma unroll
+                for (int k = 0; k < R; k++) raw[k] = acc[k][i];
+                c2_logf_clamped_n<R>(raw, lg);
+#pragma unroll
+                for (int k = 0; k < R; k++) tcur[k][i] = lg[k];
+            }
+        };
+        /* the step's R scores from the tiles: step t = 4s+u, lane 4g+j reads
+         * register u-j of the current tile or 4+u-j of the previous one */
+        auto select = [&](auto UC, float* sc) __attribute__((always_inline)) {
+            constexpr int u = decltype(UC)::value;
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const float x0 = tcur[k][u];
+                const float x1 = u >= 1 ? tcur[k][u >= 1 ? u - 1 : 0] : tprv[k][3];
+                const float x2 = u >= 2 ? tcur[k][u >= 2 ? u - 2 : 0] : tprv[k][u + 2 < 4 ? u + 2 : 3];
+                const float x3 = u >= 3 ? tcur[k][0] : tprv[k][u + 1 < 4 ? u + 1 : 3];
+                sc[k] = jl == 0 ? x0 : (jl == 1 ? x1 : (jl == 2 ? x2 : x3));
             }
         };
         read_col();
@@ -354,14 +426,24 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
              * SS term is added with weight 0 when it is off (an exact +-0) */
             float sc[R];
             {
-                float acc[R];
+                if constexpr (MF) {
+                    /* t is wave-uniform: a scalar branch picks the selection */
+                    switch (t & 3) {
+                        case 0: tile(t); select(std::integral_constant<int, 0>{}, sc); break;
+                        case 1: select(std::integral_constant<int, 1>{}, sc); break;
+                        case 2: select(std::integral_constant<int, 2>{}, sc); break;
+                        default: select(std::integral_constant<int, 3>{}, sc); break;
+                    }
+                } else {
+                    float acc[R];
 #pragma unroll
-                for (int k = 0; k < R; k++) acc[k] = q[k][0] * dw[0];
+                    for (int k = 0; k < R; k++) acc[k] = q[k][0] * dw[0];
 #pragma unroll
-                for (int j = 1; j < 20; j++)
+                    for (int j = 1; j < 20; j++)
 #pragma unroll
-                    for (int k = 0; k < R; k++) acc[k] = fmaf(q[k][j], dw[j], acc[k]);
-                c2_logf_clamped_n<R>(acc, sc);
+                        for (int k = 0; k < R; k++) acc[k] = fmaf(q[k][j], dw[j], acc[k]);
+                    c2_logf_clamped_n<R>(acc, sc);
+                }
                 const float ssw = a.ssswgt > 0.0f ? a.ssswgt : 0.0f;
                 float t0[R], t1[R], t2[R];
 #pragma unroll
@@ -468,9 +550,9 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
             soff = (soff == soff_end) ? 0 : soff + ns;
             /* the block after next: its first column is the top boundary the
              * next step reads ahead of its rows */
-            if (((t + 2) % BLK) == 0) {
-                load_block<R, CV>(a, ring, ns, lane, t + 2, c_begin, c_end, tops, s > 0,
-                              block_profile(a, lane, t + 2, c_begin, c_end), bot, bld, bc0);
+            if (((t + LEAD) % BK) == 0) {
+                load_block<R, CV, BK>(a, ring, ns, lane, t + LEAD, c_begin, c_end, tops, s > 0,
+                                      block_profile<BK>(a, lane, t + LEAD, c_begin, c_end), bot, bld, bc0);
                 wave_lds_sync();
             }
             read_col();
@@ -483,12 +565,12 @@ __device__ __forceinline__ void p1_sweep(PA& a, int seg, float* lds, float* botw
     }
 }
 
-template <int R, int NW, bool BTCK, bool CV = false>
+template <int R, int NW, bool BTCK, bool CV = false, bool MF = false>
 __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
 {
     extern __shared__ float lds[];
-    p1_sweep<R, NW, BTCK, CV, const P1Args>(a, blockIdx.x * NW + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)),
-                                            lds, nullptr, 0);
+    p1_sweep<R, NW, BTCK, CV, MF, const P1Args>(a, blockIdx.x * NW + __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6)),
+                                                lds, nullptr, 0);
 }
 
 /* persistent form for a batch of queries: workgroups pull tiles (query,
@@ -496,7 +578,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1(P1Args a)
  * tiles of one launch share the rows per lane R (one length bin) and come in
  * the order the host sorted them (largest first); every wave hands its
  * stripes over through its own scratch rows (5 x bot_ld floats per wave). */
-template <int R, int NW, bool BTCK>
+template <int R, int NW, bool BTCK, bool MF = false>
 __global__ __launch_bounds__(64 * NW, 1) void k_phase1q(P1QueueArgs qa)
 {
     extern __shared__ float lds[];
@@ -515,7 +597,7 @@ __global__ __launch_bounds__(64 * NW, 1) void k_phase1q(P1QueueArgs qa)
         /* the arguments are read-only for the whole launch: read through the
          * constant address space, as kernel arguments are */
         CP1Args* ap = (CP1Args*)(qa.args + tq);
-        p1_sweep<R, NW, BTCK, false, CP1Args>(*ap, tg * NW + wv, lds, botw, qa.bot_ld);
+        p1_sweep<R, NW, BTCK, false, MF, CP1Args>(*ap, tg * NW + wv, lds, botw, qa.bot_ld);
     }
 }
 
@@ -553,6 +635,7 @@ __global__ void k_fill_u32(uint32_t* p, int64_t n, uint32_t v)
 /* clears the phase-1 maxima with a kernel (safe inside a captured graph) */
 hipError_t c2_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t stream)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_zero_u64, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, stream, p, n);
     return hipGetLastError();
@@ -560,6 +643,7 @@ hipError_t c2_launch_zero_u64(unsigned long long* p, int64_t n, hipStream_t stre
 
 hipError_t c2_launch_weights(const float* qbkg, const float* bkg, int nprof, float* wts, hipStream_t stream)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     if (nprof <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_weights, dim3((unsigned)((nprof + 255) / 256)), dim3(256), 0, stream, qbkg, bkg, nprof, wts);
     return hipGetLastError();
@@ -568,6 +652,7 @@ hipError_t c2_launch_weights(const float* qbkg, const float* bkg, int nprof, flo
 /* clears n 32-bit words with a kernel (safe inside a captured graph) */
 hipError_t c2_launch_zero_u32(uint32_t* p, int64_t n, hipStream_t stream)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     if (n <= 0) return hipSuccess;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
@@ -577,6 +662,7 @@ hipError_t c2_launch_zero_u32(uint32_t* p, int64_t n, hipStream_t stream)
 
 hipError_t c2_launch_fill_u32(uint32_t* p, int64_t n, uint32_t v, hipStream_t stream)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     if (n <= 0) return hipSuccess;
     int64_t blocks = (n + 255) / 256;
     if (blocks > 4096) blocks = 4096;
@@ -601,33 +687,34 @@ static size_t phase1_cv_lds_bytes(int R, int NW)
 template <int RR, int NWW, bool BT>
 static hipError_t launch_cv(const P1Args& a, int grid, size_t lds, hipStream_t stream)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     C2_LDS_LIMIT_ONCE((k_phase1<RR, NWW, BT, true>), 160 * 1024);
     hipLaunchKernelGGL((k_phase1<RR, NWW, BT, true>), dim3(grid), dim3(64 * NWW), lds, stream, a);
     return hipGetLastError();
 }
 
-#define C2_P1_CASE(RR, NWW, BT)                                                         \
-    case RR:                                                                              \
-        hipLaunchKernelGGL((k_phase1<RR, NWW, BT>), dim3(grid), dim3(64 * NWW), lds, stream, a); \
+#define C2_P1_CASE(RR, NWW, BT, M)                                                         \
+    case RR:                                                                                 \
+        hipLaunchKernelGGL((k_phase1<RR, NWW, BT, false, M>), dim3(grid), dim3(64 * NWW), lds, stream, a); \
         break;
 
-template <int NWW, bool BT>
+template <int NWW, bool BT, bool M = false>
 static hipError_t launch_nw(const P1Args& a, int R, int grid, size_t lds, hipStream_t stream)
 {
     switch (R) {
-        case 1: C2_LDS_LIMIT_ONCE((k_phase1<1, NWW, BT>), 160 * 1024); break;
-        case 2: C2_LDS_LIMIT_ONCE((k_phase1<2, NWW, BT>), 160 * 1024); break;
-        case 3: C2_LDS_LIMIT_ONCE((k_phase1<3, NWW, BT>), 160 * 1024); break;
-        case 4: C2_LDS_LIMIT_ONCE((k_phase1<4, NWW, BT>), 160 * 1024); break;
-        case 5: C2_LDS_LIMIT_ONCE((k_phase1<5, NWW, BT>), 160 * 1024); break;
+        case 1: C2_LDS_LIMIT_ONCE((k_phase1<1, NWW, BT, false, M>), 160 * 1024); break;
+        case 2: C2_LDS_LIMIT_ONCE((k_phase1<2, NWW, BT, false, M>), 160 * 1024); break;
+        case 3: C2_LDS_LIMIT_ONCE((k_phase1<3, NWW, BT, false, M>), 160 * 1024); break;
+        case 4: C2_LDS_LIMIT_ONCE((k_phase1<4, NWW, BT, false, M>), 160 * 1024); break;
+        case 5: C2_LDS_LIMIT_ONCE((k_phase1<5, NWW, BT, false, M>), 160 * 1024); break;
         default: return hipErrorInvalidValue;
     }
     switch (R) {
-        C2_P1_CASE(1, NWW, BT)
-        C2_P1_CASE(2, NWW, BT)
-        C2_P1_CASE(3, NWW, BT)
-        C2_P1_CASE(4, NWW, BT)
-        C2_P1_CASE(5, NWW, BT)
+        C2_P1_CASE(1, NWW, BT, M)
+        C2_P1_CASE(2, NWW, BT, M)
+        C2_P1_CASE(3, NWW, BT, M)
+        C2_P1_CASE(4, NWW, BT, M)
+        C2_P1_CASE(5, NWW, BT, M)
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
@@ -655,6 +742,14 @@ hipError_t c2_launch_phase1(const P1Args& a, int R, int NW, bool btck, hipStream
             default: return hipErrorInvalidValue;
         }
     }
+    /* the DB sweep with the 20-term sums on the matrix cores */
+    if (a.mfma) {
+        switch (NW) {
+            case 8: return launch_nw<8, false, true>(a, R, grid, lds, stream);
+            case 4: return launch_nw<4, false, true>(a, R, grid, lds, stream);
+            default: break;
+        }
+    }
     switch (NW) {
         case 8: return launch_nw<8, false>(a, R, grid, lds, stream);
         case 4: return launch_nw<4, false>(a, R, grid, lds, stream);
@@ -668,11 +763,13 @@ static constexpr size_t Q_MAX_DYN_LDS = 160 * 1024 - 256;
 
 /* the queue kernels built: the DB sweep with 8 or 4 waves per workgroup, the
  * candidates' backtrack pass with 2 */
-static const void* phase1q_fn(int R, int NW, bool btck)
+static const void* phase1q_fn(int R, int NW, bool btck, bool mf)
 {
 #define C2_P1Q(RR)                                                                   \
     if (R == RR) {                                                                   \
         if (btck) return NW == 2 ? (const void*)k_phase1q<RR, 2, true> : nullptr;    \
+        if (mf && NW == 8) return (const void*)k_phase1q<RR, 8, false, true>;        \
+        if (mf && NW == 4) return (const void*)k_phase1q<RR, 4, false, true>;        \
         if (NW == 8) return (const void*)k_phase1q<RR, 8, false>;                    \
         if (NW == 4) return (const void*)k_phase1q<RR, 4, false>;                    \
         return nullptr;                                                              \
@@ -682,9 +779,9 @@ static const void* phase1q_fn(int R, int NW, bool btck)
     return nullptr;
 }
 
-int c2_phase1q_grid(int R, int NW, bool btck, size_t lds)
+int c2_phase1q_grid(int R, int NW, bool btck, size_t lds, bool mf)
 {
-    const void* fn = phase1q_fn(R, NW, btck);
+    const void* fn = phase1q_fn(R, NW, btck, mf);
     if (!fn || lds > Q_MAX_DYN_LDS) return 0;
     /* the kernel's own pull slot is static LDS: the dynamic limit stays below 160 KiB */
     if (hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)Q_MAX_DYN_LDS) != hipSuccess) return 0;
@@ -700,10 +797,11 @@ int c2_phase1q_grid(int R, int NW, bool btck, size_t lds)
     return per * ncu;
 }
 
-hipError_t c2_launch_phase1q(const P1QueueArgs& q, int R, int NW, bool btck, int grid, size_t lds, hipStream_t stream)
+hipError_t c2_launch_phase1q(const P1QueueArgs& q, int R, int NW, bool btck, int grid, size_t lds, hipStream_t stream,
+                             bool mf)
 {
     if (q.ntiles <= 0) return hipSuccess;
-    const void* fn = phase1q_fn(R, NW, btck);
+    const void* fn = phase1q_fn(R, NW, btck, mf);
     if (!fn || grid < 1) return hipErrorInvalidValue;
     P1QueueArgs qa = q;
     void* args[] = {&qa};

@@ -655,6 +655,7 @@ hipError_t c2_launch_finalize(const ScoreCtx& c, const int32_t* cand, int32_t nc
                               const int64_t* btck_off, FinRec* out, hipStream_t stream,
                               const int32_t* ncand_dev)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     if (ncand <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_finalize, dim3(grid_of(ncand, ncand_dev, 8192)), dim3(64), 0, stream, c, cand, ncand,
                        keys, btck, btck_off, out, ncand_dev);
@@ -663,6 +664,7 @@ hipError_t c2_launch_finalize(const ScoreCtx& c, const int32_t* cand, int32_t nc
 
 hipError_t c2_launch_aln_final(const ScoreCtx& c, const AlnArgs& a, int n, hipStream_t stream)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     if (n <= 0) return hipSuccess;
     if (!a.pro || !a.cand || !a.keys || !a.btck || !a.btck_off || !a.alns || !a.alntmp || !a.res)
         return hipErrorInvalidValue;
@@ -673,6 +675,7 @@ hipError_t c2_launch_aln_final(const ScoreCtx& c, const AlnArgs& a, int n, hipSt
 hipError_t c2_launch_hist(const ScoreCtx& c, const int32_t* pro, int32_t npro, int xuninf,
                           int32_t* counts, hipStream_t stream, const int32_t* npro_dev)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     if (npro <= 0) return hipSuccess;
     if (c.cvs.wgt > 0.0f)
         hipLaunchKernelGGL(k_hist<true>, dim3(grid_of(npro, npro_dev, 4096)), dim3(256), 0, stream, c, pro, xuninf,
@@ -686,6 +689,7 @@ hipError_t c2_launch_hist(const ScoreCtx& c, const int32_t* pro, int32_t npro, i
 hipError_t c2_launch_stats(const StatParams& sp, const float* nnw, const int32_t* counts, const int32_t* crow,
                            const StatIn* in, int32_t n, double* out, hipStream_t stream, const int32_t* n_dev)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     if (n <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_stats, dim3((n + 3) / 4), dim3(256), 0, stream, sp, nnw, counts,
                        crow, in, n, out, n_dev);
@@ -694,6 +698,7 @@ hipError_t c2_launch_stats(const StatParams& sp, const float* nnw, const int32_t
 
 hipError_t c2_launch_score_block(const ScoreCtx& c, int p, int Ld, float* S, float* mod, hipStream_t stream)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     const int64_t n = (int64_t)c.Lq * Ld;
     if (n <= 0) return hipSuccess;
     int64_t g = (n + 255) / 256;

@@ -147,7 +147,7 @@ int queue_bins(cbp_handle* h, cbp_handle::QPlan& P, int nq, bool btck,
         for (const QTileCost& t : v) P.tiles.push_back(P1Tile{t.q, t.g});
         nb.lds = c2_phase1_lds_bytes(nb.R, nb.NW, nb.tops ? 2 : 1);
         if (nb.lds > 160 * 1024) return c2_fail(h, CBP_ERR_STATE, "queue bin exceeds the LDS");
-        nb.grid = c2_phase1q_grid(nb.R, nb.NW, btck, nb.lds);
+        nb.grid = c2_phase1q_grid(nb.R, nb.NW, btck, nb.lds, !btck && h->use_mfma);
         if (nb.grid < 1) return c2_fail(h, CBP_ERR_HIP, "queue kernel occupancy");
         nb.bot0 = P.bot_floats;
         if (nb.tops) P.bot_floats += (int64_t)nb.grid * nb.NW * 5 * P.bot_ld;
@@ -179,7 +179,7 @@ int queue_launch(cbp_handle* h, cbp_handle::QPlan& P, bool btck)
         qa.head = P.dhead.as<unsigned>() + b;
         qa.botw = B.tops ? P.dbot.as<float>() + B.bot0 : nullptr;
         qa.bot_ld = P.bot_ld;
-        PCHK(c2_launch_phase1q(qa, B.R, B.NW, btck, B.grid, B.lds, slot_stream(h, b % ns)));
+        PCHK(c2_launch_phase1q(qa, B.R, B.NW, btck, B.grid, B.lds, slot_stream(h, b % ns), !btck && h->use_mfma));
     }
     for (int s = 1; s < ns; s++) {
         PCHK(hipEventRecord(h->gjoin[s - 1], slot_stream(h, s)));
@@ -748,6 +748,7 @@ int c2_queue_scan_launch(cbp_handle* h, int nq, bool timed, bool copy) { return 
 void c2_drop_graphs(cbp_handle* h)
 {
     for (auto& g : h->graphs) if (g.second.exec) (void)hipGraphExecDestroy(g.second.exec);
+    if (!h->graphs.empty()) (void)hipGetLastError();   /* unreported; not left for a launch check */
     h->graphs.clear();
 }
 
@@ -878,12 +879,14 @@ static int run_queries_guarded(cbp_handle* h, const int32_t* qidx, int32_t nq, c
 
 extern "C" int cbp_process(cbp_handle* h, int32_t qidx, float scorethld, float logevthld, cbp_results* out)
 {
+    C2ApiScope c2_scope_;
     if (!h || !out) return CBP_ERR_ARG;
     return run_queries_guarded(h, &qidx, 1, &scorethld, &logevthld, out);
 }
 
 extern "C" int cbp_process_batch(cbp_handle* h, const int32_t* qidx, int32_t nq, cbp_results* outs)
 {
+    C2ApiScope c2_scope_;
     if (!h || !qidx || !outs || nq < 0) return CBP_ERR_ARG;
     if (!h->have_opt || !h->have_queries || !h->have_chunk)
         return c2_fail(h, CBP_ERR_STATE, "options, queries and chunk must be uploaded first");
@@ -900,6 +903,7 @@ extern "C" int cbp_process_batch(cbp_handle* h, const int32_t* qidx, int32_t nq,
 
 extern "C" int cbp_set_graph(cbp_handle* h, int32_t on)
 {
+    C2ApiScope c2_scope_;
     if (!h) return CBP_ERR_ARG;
     h->use_graph = on != 0;
     drop_graphs(h);
@@ -908,6 +912,7 @@ extern "C" int cbp_set_graph(cbp_handle* h, int32_t on)
 
 extern "C" int cbp_set_device_lists(cbp_handle* h, int32_t on)
 {
+    C2ApiScope c2_scope_;
     if (!h) return CBP_ERR_ARG;
     h->use_dlists = on != 0;
     drop_graphs(h);
@@ -917,6 +922,7 @@ extern "C" int cbp_set_device_lists(cbp_handle* h, int32_t on)
 extern "C" int cbp_set_device_list_caps(cbp_handle* h, int32_t cand, int32_t passed, int64_t code_words,
                                         int64_t map_cells)
 {
+    C2ApiScope c2_scope_;
     if (!h || cand < 1 || passed < 1 || passed > DL_SORT || code_words < 64 || map_cells < 64) return CBP_ERR_ARG;
     h->dl_capC = cand; h->dl_capP = passed; h->dl_capW = code_words; h->dl_capCells = map_cells;
     drop_graphs(h);
@@ -925,6 +931,7 @@ extern "C" int cbp_set_device_list_caps(cbp_handle* h, int32_t cand, int32_t pas
 
 extern "C" int cbp_last_fallbacks(const cbp_handle* h, int32_t* n)
 {
+    C2ApiScope c2_scope_;
     if (!h || !n) return CBP_ERR_ARG;
     *n = h->last_dl_fallbacks;
     return CBP_OK;
@@ -932,8 +939,27 @@ extern "C" int cbp_last_fallbacks(const cbp_handle* h, int32_t* n)
 
 extern "C" int cbp_set_queue(cbp_handle* h, int32_t on)
 {
+    C2ApiScope c2_scope_;
     if (!h) return CBP_ERR_ARG;
     h->use_queue = on != 0;
+    drop_graphs(h);
+    return CBP_OK;
+}
+
+extern "C" int cbp_set_map_rows(cbp_handle* h, int32_t rmax)
+{
+    C2ApiScope c2_scope_;
+    if (!h || rmax < 0 || rmax > 5) return CBP_ERR_ARG;
+    h->map_rmax2 = rmax;
+    drop_graphs(h);
+    return CBP_OK;
+}
+
+extern "C" int cbp_set_mfma(cbp_handle* h, int32_t on)
+{
+    C2ApiScope c2_scope_;
+    if (!h) return CBP_ERR_ARG;
+    h->use_mfma = on != 0;
     drop_graphs(h);
     return CBP_OK;
 }

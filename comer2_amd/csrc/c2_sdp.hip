@@ -735,6 +735,7 @@ int64_t c2_sdp_ckpt_floats(int R) { return (int64_t)(5 * R + 10) * 64; }
 
 hipError_t c2_launch_sdp(const SdpArgs& a, bool adjoint, int groups, int nw, hipStream_t stream)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     if (a.npairs <= 0) return hipSuccess;
     if (groups < 1 || nw < 1 || nw > 8) return hipErrorInvalidValue;
     if (a.K < 0 || (a.K > 0 && 64 % a.K != 0)) return hipErrorInvalidValue;
@@ -755,6 +756,7 @@ hipError_t c2_launch_sdp(const SdpArgs& a, bool adjoint, int groups, int nw, hip
 
 hipError_t c2_launch_sdp_scores(const SdpScoreArgs& a, int npairs, hipStream_t stream)
 {
+    (void)hipGetLastError();   /* the status read after the launches is theirs alone */
     if (npairs <= 0) return hipSuccess;
     hipLaunchKernelGGL(k_sdp_scores, dim3(npairs), dim3(256), 0, stream, a);
     return hipGetLastError();

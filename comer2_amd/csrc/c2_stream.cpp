@@ -473,6 +473,7 @@ int c2_search_db(cbp_handle* h, const cbp_profiles* db, uint64_t chunk_pos, cons
 extern "C" int cbp_search_db(cbp_handle* h, const cbp_profiles* db, uint64_t chunk_pos, const int32_t* qidx,
                              int32_t nq, cbp_results* outs)
 {
+    C2ApiScope c2_scope_;
     if (!h || !db || !qidx || !outs || nq < 1) return CBP_ERR_ARG;
     try {
         return c2_search_db(h, db, chunk_pos, qidx, nq, outs);
@@ -483,6 +484,7 @@ extern "C" int cbp_search_db(cbp_handle* h, const cbp_profiles* db, uint64_t chu
 
 extern "C" int cbp_last_search_chunks(const cbp_handle* h, int32_t* nchunks)
 {
+    C2ApiScope c2_scope_;
     if (!h || !nchunks) return CBP_ERR_ARG;
     *nchunks = h->last_nchunks;
     return CBP_OK;

@@ -27,14 +27,7 @@ def lib():
         return _lib
     if not os.path.exists(_ffi.LIB_HIP):
         raise CbpError("libcomer2_hip.so is not built (run comer2_amd.build)")
-    # PyTorch ships its own HIP runtime with the same soname (libamdhip64.so.7);
-    # loading it first makes this library bind to it, so a process that also
-    # uses torch GPU tensors has one HIP runtime, not two.
-    try:
-        import torch  # noqa: F401
-    except ImportError:
-        pass
-    L = C.CDLL(_ffi.LIB_HIP)
+    L = _ffi.load_hip()
     H = C.c_void_p
     L.cbp_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.POINTER(H)]
     L.cbp_destroy.argtypes = [H]
@@ -53,6 +46,8 @@ def lib():
     L.cbp_set_tuning.argtypes = [H, C.c_int32, C.c_int32, C.c_int32]
     L.cbp_set_graph.argtypes = [H, C.c_int32]
     L.cbp_set_queue.argtypes = [H, C.c_int32]
+    L.cbp_set_mfma.argtypes = [H, C.c_int32]
+    L.cbp_set_map_rows.argtypes = [H, C.c_int32]
     L.cbp_set_device_lists.argtypes = [H, C.c_int32]
     L.cbp_set_device_list_caps.argtypes = [H, C.c_int32, C.c_int32, C.c_int64, C.c_int64]
     L.cbp_last_fallbacks.argtypes = [H, C.POINTER(C.c_int32)]
@@ -246,6 +241,16 @@ class CuBatchProcessing:
         backtrack pass from the device work queue (on, the default) or one
         launch per query (off)."""
         self._check(self.L.cbp_set_queue(self.h, 1 if on else 0))
+
+    def set_mfma(self, on=True):
+        """The phase-1 DB sweep forms the 20-term score sums on the matrix
+        cores (on) or the vector ALUs (off, the default); bit-identical."""
+        self._check(self.L.cbp_set_mfma(self.h, 1 if on else 0))
+
+    def set_map_rows(self, rmax):
+        """MAP passes of one-query calls: the shorter pairs' group takes at
+        most rmax rows per lane (1-5), or the long pairs' geometry (0)."""
+        self._check(self.L.cbp_set_map_rows(self.h, int(rmax)))
 
     def autotune(self, qidx=0):
         """Times phase 1 of query qidx over segment widths, waves per workgroup

@@ -32,7 +32,7 @@ class PlainInfo(C.Structure):
 def _L():
     global _lib
     if _lib is None:
-        L = C.CDLL(LIB_HIP)
+        L = _ffi.load_hip()
         for f in (L.cbp_format_plain, L.cbp_format_json):
             f.argtypes = [C.POINTER(Options), C.POINTER(PlainInfo), C.POINTER(Results),
                           C.c_char_p, C.c_size_t, C.POINTER(C.c_size_t)]

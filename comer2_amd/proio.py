@@ -11,7 +11,7 @@ and ``desc(i)`` / ``name(i)`` the description and file name of profile i
 """
 import ctypes as C
 import numpy as np
-from ._ffi import LIB_HIP, Profiles
+from ._ffi import LIB_HIP, Profiles, load_hip  # noqa: F401
 
 _lib = None
 
@@ -19,7 +19,7 @@ _lib = None
 def _L():
     global _lib
     if _lib is None:
-        L = C.CDLL(LIB_HIP)
+        L = load_hip()
         V = C.c_void_p
         L.cbp_proset_create.argtypes = [C.POINTER(V)]
         L.cbp_proset_read_text.argtypes = [V, C.c_char_p]

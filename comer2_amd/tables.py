@@ -68,8 +68,8 @@ def load_cvs2s(path=None):
     """The context-vector map (src/env/cvs2s) as the library reads it
     (cbp_cvs2s_read): (keys, smoothed scores, shift).  Host only."""
     path = path or os.path.join(_DATA, "cvs2s")
-    from ._ffi import LIB_HIP
-    L = C.CDLL(LIB_HIP)
+    from ._ffi import load_hip
+    L = load_hip()
     L.cbp_cvs2s_read.argtypes = [C.c_char_p, C.c_void_p, C.c_void_p, C.c_int32,
                                  C.POINTER(C.c_int32), C.POINTER(C.c_int32)]
     card, shift = C.c_int32(), C.c_int32()

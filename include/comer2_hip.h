@@ -379,6 +379,26 @@ int  cbp_set_graph(cbp_handle* h, int32_t on);
  * cualn/DevCommThread.cpp:296-356) */
 int  cbp_set_queue(cbp_handle* h, int32_t on);
 
+/* the phase-1 DB sweep (one-query launches and the batch work queue) forms
+ * each cell's 20-term score sum on the matrix cores (on = 1) instead of the
+ * vector ALUs (on = 0): every group of four lanes computes its rows against
+ * four columns with v_mfma_f32_4x4x1_16b_f32 chained over the 20 terms, one
+ * product added per instruction, which rounds as the fmaf chain does, so
+ * keys and every later stage are bit-identical.  The backtrack pass and the
+ * context-vector sweep keep the vector form.  Replaces the score-matrix
+ * contraction of CalcSM_Init_SMEMUnroll2x (cusco/CuBatchScoreMatrix_Init.cu:45-186) */
+int  cbp_set_mfma(cbp_handle* h, int32_t on);
+
+/* MAP passes of a one-query call (the pairs no longer than half the longest
+ * run as their own group beside the long pairs): rmax = 1..5 gives that
+ * group at most rmax query rows per lane and as few stripes (waves per pair)
+ * as that allows, so each wave works on more rows and fewer waves wait at
+ * the band's per-step barriers; rmax = 0 keeps the long pairs' geometry
+ * (one band of up to 8 waves, R = ceil(Lq/512)) for every pair.  Results are
+ * bit-identical either way.  Replaces the fixed 32x32-block schedule of
+ * ExecMAPDP_Parent_DynPlm (cumapdp/CuBatchMAPDP_prnt.cu:18-115) */
+int  cbp_set_map_rows(cbp_handle* h, int32_t rmax);
+
 /* stage lists built on the device (on = 1, the default): the candidate,
  * pass and drawing lists, the (log E, profile) order and the MAP plan of
  * cbp_process / cbp_process_batch are ordered compactions and sorts on the

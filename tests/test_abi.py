@@ -27,7 +27,7 @@ def _declared():
 
 
 def test_every_declared_symbol_is_exported():
-    L = C.CDLL(_ffi.LIB_HIP)
+    L = _ffi.load_hip()
     names = _declared()
     assert len(names) >= 15
     missing = [n for n in names if not hasattr(L, n)]
@@ -35,7 +35,7 @@ def test_every_declared_symbol_is_exported():
 
 
 def test_argument_errors_are_codes():
-    L = C.CDLL(_ffi.LIB_HIP)
+    L = _ffi.load_hip()
     L.cbp_destroy.argtypes = [C.c_void_p]
     L.cbp_last_error.restype = C.c_char_p
     L.cbp_last_error.argtypes = [C.c_void_p]
@@ -51,7 +51,7 @@ def test_create_without_gpu_reports_nodev():
     import torch
     if torch.cuda.device_count() > 0:
         pytest.skip("a GPU is present")
-    L = C.CDLL(_ffi.LIB_HIP)
+    L = _ffi.load_hip()
     h = C.c_void_p()
     L.cbp_create.argtypes = [C.c_int, C.c_size_t, C.c_int, C.c_void_p]
     assert L.cbp_create(0, 0, 0, C.byref(h)) == -5
@@ -60,7 +60,7 @@ def test_create_without_gpu_reports_nodev():
 
 @pytest.mark.parametrize("evalue", [10.0, 1.0, 1e-3, 0.0])
 def test_score_threshold_matches_oracle(evalue):
-    L = C.CDLL(_ffi.LIB_HIP)
+    L = _ffi.load_hip()
     L.cbp_score_threshold.argtypes = [C.POINTER(_ffi.Options), C.c_uint64, C.c_uint64,
                                       C.POINTER(C.c_float), C.POINTER(C.c_float)]
     opts = dict(tables.default_options(), evalue=evalue)

@@ -25,7 +25,7 @@ PP = C.c_void_p * NF
 
 
 def _libs():
-    L = C.CDLL(_ffi.LIB_HIP)
+    L = _ffi.load_hip()
     T = C.CDLL(os.path.join(ROOT, "tests", "libadapter_check.so"))
     T.c2t_layout.restype = C.c_void_p
     T.c2t_layout.argtypes = [C.POINTER(_ffi.Profiles), C.c_int, C.POINTER(C.c_size_t), PP, PP]

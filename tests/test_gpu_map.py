@@ -134,3 +134,31 @@ def test_map_score_chunk_edges():
             n += 1
             seen.add(ED[dp])
     assert n > 0 and seen == set(ED)
+
+
+def _same(a, b):
+    return (a["npassed"] == b["npassed"] and a["nmap"] == b["nmap"]
+            and np.array_equal(a["alndata"].view(np.uint32), b["alndata"].view(np.uint32))
+            and np.array_equal(a["order"], b["order"]) and np.array_equal(a["aln_raw"], b["aln_raw"]))
+
+
+@pytest.mark.parametrize("qlen", [(60, 120), (250, 330), (560, 640), (1000, 1100)])
+def test_map_shorter_group_geometry(qlen):
+    """One-query calls realign the pairs no longer than half the longest as
+    their own group; whatever rows per lane that group takes (1-5, or the
+    long pairs' geometry), every record, order and string is the same."""
+    q = SynthDB(81, 1, 0, 0, qlen[0], qlen[1])
+    d = SynthDB(82, 120, 160, 0.668, 40, 1500)
+    d.plant(q, 83, 30, 3.0, 60.0)
+    cbp = processing()
+    cbp.TransferQueries(q)
+    cbp.TransferCPMData(d)
+    cbp.SetDbDetails(d.npos, d.nprof)
+    out = {}
+    for rmax in (0, 1, 2, 3, 4, 5):
+        cbp.set_map_rows(rmax)
+        print(f"map shorter group: Lq={int(q.lengths()[0])} rmax={rmax}", flush=True)
+        out[rmax] = cbp.ProcessScoreMatrix(0, scorethld=6.0, logevthld=np.log(1e-2))
+    assert out[0]["nmap"] >= 3
+    for rmax in (1, 2, 3, 4, 5):
+        assert _same(out[0], out[rmax]), rmax
