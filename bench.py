@@ -365,6 +365,9 @@ def main():
     ap.add_argument("--no-stream", action="store_true")
     ap.add_argument("--map-rows", type=int, default=-1,
                     help="rows per lane of the shorter pairs' MAP group (0-5; default: the library's)")
+    ap.add_argument("--map-pipe", type=int, default=-1,
+                    help="MAP passes streaming pairs through one band per workgroup (1) or one workgroup "
+                         "per pair (0); default: the library's")
     ap.add_argument("--phase1-variants", action="store_true",
                     help="only time the phase-1 sweep of the C1 query with its score sums on the vector "
                          "ALUs and on the matrix cores (and a 256-row query, one stripe at 4 rows per lane), "
@@ -396,6 +399,8 @@ def main():
     cbp = CuBatchProcessing(0, 0, 0, options=dict(max_map_pairs=MAP_CAP))
     if args.map_rows >= 0:
         cbp.set_map_rows(args.map_rows)
+    if args.map_pipe >= 0:
+        cbp.set_map_pipe(args.map_pipe != 0)
     t = time.perf_counter()
     cbp.TransferQueries(q)
     cbp.TransferCPMData(d)

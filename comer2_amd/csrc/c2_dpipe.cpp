@@ -232,6 +232,7 @@ int dl_chain(cbp_handle* h, int i, const DlPlan& P)
         ma.order = sl.morder.as<int32_t>(); ma.R = mR; ma.ns = mns; ma.nw = mnw;
         ma.sitems = sl.sitems.as<int4>(); ma.nsitems = (int32_t)std::min<int64_t>(k.capItems, 1 << 30);
         ma.dcnt = q.cnt.as<int32_t>();
+        ma.pipe = h->map_pipe ? 1 : 0;
         if (P.split) {
             /* the long-pair group on a stream of its own, each group's
              * backward pass beside its forward pass */

@@ -125,6 +125,9 @@ struct MapArgs {
     int32_t npairs = 0;
     const int32_t* dcnt = nullptr;
     int32_t group = -1;
+    /* forward and backward passes with one band per workgroup streaming its
+     * pairs back to back (queries whose stripes fit one band) */
+    int32_t pipe = 0;
 };
 /* columns per item of the dynamic-score pass */
 constexpr int C2_MAP_SCORE_COLS = 256;

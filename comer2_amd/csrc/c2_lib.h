@@ -264,6 +264,9 @@ struct cbp_handle {
      * longest take at most this many rows per lane (0: the long pairs'
      * geometry for every pair) */
     int map_rmax2 = 0;
+    /* MAP forward and backward passes stream a workgroup's pairs through one
+     * band (queries whose stripes fit 8 waves) */
+    bool map_pipe = true;
     QPlan qscan, qcand;
     DBuf qkeys, qwts;                   /* [nq][nprof] maxima, [nq][nprof][20] weights */
     unsigned long long* hqkeys = nullptr; size_t hqkeys_n = 0;

@@ -425,6 +425,146 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(R == 1 ? 4 
     }
 }
 
+/* ---- forward, one band streaming its pairs back to back ----
+ * For a query whose stripes fit one band (ns <= 8 waves): a persistent
+ * workgroup takes the pairs at positions b, b+G, b+2G, ... of the launch
+ * order (G = the grid) and runs them through its band without draining it:
+ * wave w starts a pair's sweep when its own sweep of the previous pair ends,
+ * 64 steps after wave w-1 does, so no wave idles between pairs and the band
+ * fills and drains once per workgroup instead of once per pair.  A pair i
+ * occupies wave w's steps u = T - 64w in [S_i, S_i + Ld_i + 64) (the first
+ * only takes the row above), S_{i+1} = S_i + Ld_i + 64, the same for every
+ * wave; when wave w needs the row above at a column of pair i, wave w-1 is
+ * 64 steps further in the same pair, so the LDS hand-off is the per-pair
+ * kernel's.  Every cell is computed as there; the pair's max F is the max of
+ * the waves' maxima (non-negative, so an integer atomicMax on its bits). */
+template <int R>
+__global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(R == 1 ? 4 : 1))) void k_map_fwd_pipe(ScoreCtx c, MapArgs m)
+{
+    __shared__ float hx[2][8][5];
+    __shared__ __align__(16) float tab[512];
+    load_lse_tab(tab);
+    int o0_, n_;
+    map_range(m, &o0_, &n_);
+    const int G = (int)gridDim.x;
+    if ((int)blockIdx.x >= n_) return;            /* uniform for the workgroup */
+    const int lane = threadIdx.x & 63, NW = blockDim.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int Lq = c.Lq, s = w;
+    const size_t ntr = (size_t)(c.npos + c.nprof);
+    /* the band's length and the pairs' maxima cleared (each pair belongs to
+     * this workgroup only) */
+    int TT = 64 * (NW - 1);
+    for (int j = (int)blockIdx.x; j < n_; j += G) {
+        const int pj = m.order[o0_ + j];
+        TT += c.len[m.pairs[pj].pro] + 64;
+        if (threadIdx.x == 0) m.stat[(size_t)pj * 4 + 0] = 0.0f;
+    }
+    __syncthreads();
+    int r[R]; bool valid[R]; float qt[R][7];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        r[k] = s * 64 * R + R * lane + k;
+        valid[k] = r[k] < Lq;
+#pragma unroll
+        for (int j = 0; j < 7; j++) qt[k][j] = valid[k] ? m.qtr[(size_t)j * c.Lpad + r[k]] : 0.0f;
+    }
+    /* the wave's current pair */
+    int jc = (int)blockIdx.x, S = 0, pi = 0, Ld = 0, nsteps = 0;
+    size_t db = 0;
+    const float* Sd = nullptr;
+    float* F = nullptr;
+    auto pair_at = [&](int j) {
+        pi = m.order[o0_ + j];
+        const MapPair mp = m.pairs[pi];
+        Ld = c.len[mp.pro]; nsteps = Ld + 63;
+        db = (size_t)c.dist[mp.pro] + mp.pro;
+        Sd = m.Sd + mp.foff; F = m.F + mp.foff;
+    };
+    pair_at(jc);
+    float mymax = 0.0f;
+    S5 H[R];
+    S5 U = init5(), D = init5();
+    float sdn[R], dtn[7];
+    for (int T = 0; T < TT; T++) {
+        const int u = T - 64 * w;
+        if (u == S + nsteps + 1 && jc < n_) {
+            jc += G; S = u;
+            if (jc < n_) pair_at(jc);
+        }
+        const bool act = u >= S && jc < n_;
+        const int t = u - S - 1;                 /* -1 only takes the row above */
+        if (act && t == -1) {
+#pragma unroll
+            for (int k = 0; k < R; k++) H[k] = init5();
+            U = init5(); D = init5();
+            mymax = 0.0f;
+#pragma unroll
+            for (int k = 0; k < R; k++) sdn[k] = Sd[lsi<R>(s, 0, k, lane, nsteps)];
+            fwd_trn(m.trans, ntr, db, -lane, Ld, dtn);
+        }
+        if (act && t >= 0 && t < nsteps) {
+            const int cc = t - lane;
+            float sdc[R], dt[7];
+#pragma unroll
+            for (int k = 0; k < R; k++) sdc[k] = sdn[k];
+#pragma unroll
+            for (int j = 0; j < 7; j++) dt[j] = dtn[j];
+            if (t + 1 < nsteps) {
+#pragma unroll
+                for (int k = 0; k < R; k++) sdn[k] = Sd[lsi<R>(s, t + 1, k, lane, nsteps)];
+                fwd_trn(m.trans, ntr, db, cc + 1, Ld, dtn);
+            }
+            if (cc >= 0 && cc < Ld) {
+                S5 prevk = D;
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    S5 lf = H[k], dg = prevk;
+                    if (cc == 0) {
+                        const int rr = valid[k] ? r[k] : 0;
+                        lf.mm = m.mleft[0 * (size_t)Lq + rr]; lf.mi = m.mleft[1 * (size_t)Lq + rr];
+                        lf.im = m.mleft[2 * (size_t)Lq + rr]; lf.dn = m.mleft[3 * (size_t)Lq + rr];
+                        lf.nd = m.mleft[4 * (size_t)Lq + rr];
+                        dg.mm = m.mdiag[0 * (size_t)Lq + rr]; dg.mi = m.mdiag[1 * (size_t)Lq + rr];
+                        dg.im = m.mdiag[2 * (size_t)Lq + rr]; dg.dn = m.mdiag[3 * (size_t)Lq + rr];
+                        dg.nd = m.mdiag[4 * (size_t)Lq + rr];
+                    }
+                    const S5 up = (k == 0) ? U : H[k - 1];
+                    prevk = H[k];
+                    const int64_t ix = lsi<R>(s, t, k, lane, nsteps);
+                    const S5 n = fcell(dg, up, lf, qt[k], dt, sdc[k], tab);
+                    H[k] = n;
+                    if (valid[k]) { F[ix] = n.mm; mymax = fmx(mymax, n.mm); }
+                }
+                if (lane == 63 && w + 1 < NW) {
+                    float* o = hx[T & 1][w];
+                    o[0] = H[R - 1].mm; o[1] = H[R - 1].mi; o[2] = H[R - 1].im;
+                    o[3] = H[R - 1].dn; o[4] = H[R - 1].nd;
+                }
+            }
+            if (t == nsteps - 1) {
+                /* the wave's sweep of the pair ends: its maximum */
+                float mx = mymax;
+                for (int off = 32; off >= 1; off >>= 1) mx = fmx(mx, __shfl_xor(mx, off));
+                if (lane == 0) atomicMax((unsigned int*)(m.stat + (size_t)pi * 4 + 0), __float_as_uint(mx));
+            }
+        }
+        __syncthreads();
+        if (act && t >= -1 && t + 1 < nsteps) {
+            /* the row above the lane's first row at column t+1 - lane */
+            S5 tp = init5();
+            if (lane == 0 && w > 0 && t + 1 < Ld) {
+                const float* o = hx[T & 1][w - 1];
+                tp.mm = o[0]; tp.mi = o[1]; tp.im = o[2]; tp.dn = o[3]; tp.nd = o[4];
+            }
+            D = U;
+            U.mm = wave_shr1(H[R - 1].mm, tp.mm); U.mi = wave_shr1(H[R - 1].mi, tp.mi);
+            U.im = wave_shr1(H[R - 1].im, tp.im); U.dn = wave_shr1(H[R - 1].dn, tp.dn);
+            U.nd = wave_shr1(H[R - 1].nd, tp.nd);
+        }
+    }
+}
+
 /* total forward probability in the reference's block order
  * (MAPDP_UpdateForTotalProbability, CuBatchMAPDP_bwd.cuh:169-231) */
 __device__ void blk_origin(int dd, int s, int lastdiag, int* x0, int* y) {
@@ -651,6 +791,126 @@ __global__ __launch_bounds__(512) void k_map_bwd(ScoreCtx c, MapArgs m)
     }
 }
 
+/* ---- backward, one band streaming its pairs back to back ----
+ * The reverse sweep of k_map_bwd with the pair stream of k_map_fwd_pipe:
+ * wave w runs 64 steps behind wave w+1 (the stripe below), and a pair i
+ * occupies wave w's steps u = T - 64(NW-1-w) in [S_i, S_i + Ld_i + 64), step
+ * t = Ld_i + 63 - (u - S_i) (the first only takes the row below). */
+template <int R>
+__global__ __launch_bounds__(512) void k_map_bwd_pipe(ScoreCtx c, MapArgs m)
+{
+    __shared__ float hb[2][8][3];
+    __shared__ __align__(16) float tab[512];
+    load_lse_tab(tab);
+    int o0_, n_;
+    map_range(m, &o0_, &n_);
+    const int G = (int)gridDim.x;
+    if ((int)blockIdx.x >= n_) return;            /* uniform for the workgroup */
+    const int lane = threadIdx.x & 63, NW = blockDim.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int Lq = c.Lq, s = w, ns = NW;
+    const size_t ntr = (size_t)(c.npos + c.nprof);
+    int TT = 64 * (NW - 1);
+    for (int j = (int)blockIdx.x; j < n_; j += G) TT += c.len[m.pairs[m.order[o0_ + j]].pro] + 64;
+    int r[R]; bool valid[R]; float qt[R][7];
+#pragma unroll
+    for (int k = 0; k < R; k++) {
+        r[k] = s * 64 * R + R * lane + k;
+        valid[k] = r[k] < Lq;
+#pragma unroll
+        for (int j = 0; j < 7; j++) qt[k][j] = valid[k] ? m.qtrb[(size_t)j * c.Lpad + r[k]] : 0.0f;
+    }
+    const bool below = (s + 1 < ns);
+    int jc = (int)blockIdx.x, S = 0, Ld = 0, nsteps = 0;
+    size_t db = 0;
+    const float* Sd = nullptr;
+    float* Bm = nullptr;
+    auto pair_at = [&](int j) {
+        const MapPair mp = m.pairs[m.order[o0_ + j]];
+        Ld = c.len[mp.pro]; nsteps = Ld + 63;
+        db = (size_t)c.dist[mp.pro] + mp.pro;
+        Sd = m.Sd + mp.foff; Bm = m.B + mp.foff;
+    };
+    pair_at(jc);
+    S5 cur[R];
+    float scur[R];
+    float shmm = 0.0f, shmi = LP0, shdn = LP0, shs = LP0;
+    float spmm = 0.0f, sps = LP0;
+    float sdn[R], dxn[7];
+    for (int T = 0; T < TT; T++) {
+        const int u = T - 64 * (NW - 1 - w);
+        if (u == S + nsteps + 1 && jc < n_) {
+            jc += G; S = u;
+            if (jc < n_) pair_at(jc);
+        }
+        const bool act = u >= S && jc < n_;
+        const int t = nsteps - (u - S);          /* nsteps only takes the row below */
+        if (act && t == nsteps) {
+#pragma unroll
+            for (int k = 0; k < R; k++) { cur[k] = init5(); scur[k] = LP0; }
+            shmm = 0.0f; shmi = LP0; shdn = LP0; shs = LP0;
+            spmm = 0.0f; sps = LP0;
+#pragma unroll
+            for (int k = 0; k < R; k++) sdn[k] = Sd[lsi<R>(s, nsteps - 1, k, lane, nsteps)];
+            bwd_trn(m.trans, ntr, db, nsteps - 1 - lane, Ld, dxn);
+        }
+        if (act && t >= 0 && t < nsteps) {
+            const int cc = t - lane;
+            float sdc[R], dx[7];
+#pragma unroll
+            for (int k = 0; k < R; k++) sdc[k] = sdn[k];
+#pragma unroll
+            for (int j = 0; j < 7; j++) dx[j] = dxn[j];
+            if (t > 0) {
+#pragma unroll
+                for (int k = 0; k < R; k++) sdn[k] = Sd[lsi<R>(s, t - 1, k, lane, nsteps)];
+                bwd_trn(m.trans, ntr, db, cc - 1, Ld, dxn);
+            }
+            if (cc >= 0 && cc < Ld) {
+                float nmi = shmi, ndn = shdn;
+                float omm = spmm, os = sps;
+#pragma unroll
+                for (int k = R - 1; k >= 0; k--) {
+                    const float dgmm = omm, dsc = os;
+                    const S5 rt = cur[k];
+                    omm = cur[k].mm; os = scur[k];
+                    const int64_t ix = lsi<R>(s, t, k, lane, nsteps);
+                    const float sk = sdc[k];
+                    S5 n = init5();
+                    if (valid[k]) {
+                        n = bcell(dgmm, nmi, ndn, rt, qt[k], dx, dsc, tab);
+                        Bm[ix] = n.mm;
+                    }
+                    cur[k] = n; scur[k] = sk;
+                    nmi = n.mi; ndn = n.dn;
+                }
+                if (lane == 0 && w > 0) {
+                    float* o = hb[T & 1][w];
+                    o[0] = cur[0].mm; o[1] = cur[0].mi; o[2] = cur[0].dn;
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < R; k++) { cur[k] = init5(); scur[k] = LP0; }
+            }
+        }
+        __syncthreads();
+        if (act && t >= 1 && t <= nsteps) {
+            float obmm = 0.0f, obmi = LP0, obdn = LP0, obs = LP0;
+            const int cb = t - 1 - 63;
+            if (lane == 63 && below && cb >= 0 && cb < Ld) {
+                const float* o = hb[T & 1][w + 1];
+                obmm = o[0]; obmi = o[1]; obdn = o[2];
+                obs = Sd[lsi<R>(s + 1, cb, 0, 0, nsteps)];
+            }
+            spmm = shmm; sps = shs;
+            shmm = wave_shl1(cur[0].mm, obmm);
+            shmi = wave_shl1(cur[0].mi, obmi);
+            shdn = wave_shl1(cur[0].dn, obdn);
+            shs = wave_shl1(scur[0], obs);
+        }
+    }
+}
+
 __device__ __forceinline__ bool in_valid(int Lq, int Ld, int e, int y, int x) {
     const bool corner = (Lq <= y + e && x <= e + y - Lq) || (Ld - e + y <= x && y < e);
     return !corner && 0 <= y && y < Lq && 0 <= x && x < Ld;
@@ -806,6 +1066,20 @@ __global__ __launch_bounds__(64) void k_map_final(ScoreCtx c, MapArgs m, int npa
     (void)npairs;
 }
 
+/* persistent grid of the streaming band kernels: the workgroups that fit
+ * the device at once (at most npairs) */
+template <class K>
+int pipe_grid(K kern, int nw, int npairs)
+{
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess || ncu < 1 ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kern, 64 * nw, 0) != hipSuccess || per < 1)
+        return npairs;
+    const int g = per * ncu;
+    return g < npairs ? g : npairs;
+}
+
 template <int R>
 hipError_t launch_r(const ScoreCtx& c, const MapArgs& m0, int npairs, hipStream_t stream,
                     hipStream_t bstream, hipEvent_t fork, hipEvent_t join)
@@ -824,18 +1098,28 @@ hipError_t launch_r(const ScoreCtx& c, const MapArgs& m0, int npairs, hipStream_
         if (c.cvs.wgt > 0.0f) hipLaunchKernelGGL((k_map_score<R, true>), dim3(grid_s), dim3(256), 0, stream, c, m);
         else hipLaunchKernelGGL((k_map_score<R, false>), dim3(grid_s), dim3(256), 0, stream, c, m);
     }
+    /* one band per query (its stripes fit 8 waves): the forward and backward
+     * passes stream each workgroup's pairs through its band */
+    const bool pipe = m.pipe && m.nw == m.ns;
+    const int gf = pipe ? pipe_grid(k_map_fwd_pipe<R>, m.nw, npairs) : npairs;
+    const int gb = pipe ? pipe_grid(k_map_bwd_pipe<R>, m.nw, npairs) : npairs;
+    auto bwd = [&](hipStream_t st) {
+        if (pipe) hipLaunchKernelGGL((k_map_bwd_pipe<R>), dim3(gb), dim3(wg), 0, st, c, m);
+        else hipLaunchKernelGGL((k_map_bwd<R>), dim3(npairs), dim3(wg), 0, st, c, m);
+    };
     if (bstream) {
         if ((e = hipEventRecord(fork, stream)) != hipSuccess) return e;
         if ((e = hipStreamWaitEvent(bstream, fork, 0)) != hipSuccess) return e;
-        hipLaunchKernelGGL((k_map_bwd<R>), dim3(npairs), dim3(wg), 0, bstream, c, m);
+        bwd(bstream);
         if ((e = hipEventRecord(join, bstream)) != hipSuccess) return e;
     }
-    hipLaunchKernelGGL((k_map_fwd<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
+    if (pipe) hipLaunchKernelGGL((k_map_fwd_pipe<R>), dim3(gf), dim3(wg), 0, stream, c, m);
+    else hipLaunchKernelGGL((k_map_fwd<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
     hipLaunchKernelGGL((k_map_total<R>), dim3(npairs), dim3(256), 0, stream, c, m);
     if (bstream) {
         if ((e = hipStreamWaitEvent(stream, join, 0)) != hipSuccess) return e;
     } else {
-        hipLaunchKernelGGL((k_map_bwd<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
+        bwd(stream);
     }
     hipLaunchKernelGGL((k_map_dp<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
     hipLaunchKernelGGL((k_map_final<R>), dim3(npairs), dim3(64), 0, stream, c, m, npairs);

@@ -946,6 +946,15 @@ extern "C" int cbp_set_queue(cbp_handle* h, int32_t on)
     return CBP_OK;
 }
 
+extern "C" int cbp_set_map_pipe(cbp_handle* h, int32_t on)
+{
+    C2ApiScope c2_scope_;
+    if (!h) return CBP_ERR_ARG;
+    h->map_pipe = on != 0;
+    drop_graphs(h);
+    return CBP_OK;
+}
+
 extern "C" int cbp_set_map_rows(cbp_handle* h, int32_t rmax)
 {
     C2ApiScope c2_scope_;

@@ -48,6 +48,7 @@ def lib():
     L.cbp_set_queue.argtypes = [H, C.c_int32]
     L.cbp_set_mfma.argtypes = [H, C.c_int32]
     L.cbp_set_map_rows.argtypes = [H, C.c_int32]
+    L.cbp_set_map_pipe.argtypes = [H, C.c_int32]
     L.cbp_set_device_lists.argtypes = [H, C.c_int32]
     L.cbp_set_device_list_caps.argtypes = [H, C.c_int32, C.c_int32, C.c_int64, C.c_int64]
     L.cbp_last_fallbacks.argtypes = [H, C.POINTER(C.c_int32)]
@@ -251,6 +252,11 @@ class CuBatchProcessing:
         """MAP passes of one-query calls: the shorter pairs' group takes at
         most rmax rows per lane (1-5), or the long pairs' geometry (0)."""
         self._check(self.L.cbp_set_map_rows(self.h, int(rmax)))
+
+    def set_map_pipe(self, on=True):
+        """MAP forward and backward passes stream each workgroup's pairs
+        through one band (on, the default) or run one workgroup per pair."""
+        self._check(self.L.cbp_set_map_pipe(self.h, 1 if on else 0))
 
     def autotune(self, qidx=0):
         """Times phase 1 of query qidx over segment widths, waves per workgroup

@@ -399,6 +399,14 @@ int  cbp_set_mfma(cbp_handle* h, int32_t on);
  * ExecMAPDP_Parent_DynPlm (cumapdp/CuBatchMAPDP_prnt.cu:18-115) */
 int  cbp_set_map_rows(cbp_handle* h, int32_t rmax);
 
+/* MAP forward and backward passes (on = 1, the default): for queries whose
+ * stripes fit one band of 8 waves, persistent workgroups stream their pairs
+ * through the band back to back, so waves do not idle while a pair's band
+ * fills and drains; on = 0 runs one workgroup per pair.  Results are
+ * bit-identical either way.  Replaces the per-pair launch loops of
+ * ExecMAPDP_Parent_DynPlm (cumapdp/CuBatchMAPDP_prnt.cu:18-115) */
+int  cbp_set_map_pipe(cbp_handle* h, int32_t on);
+
 /* stage lists built on the device (on = 1, the default): the candidate,
  * pass and drawing lists, the (log E, profile) order and the MAP plan of
  * cbp_process / cbp_process_batch are ordered compactions and sorts on the

@@ -162,3 +162,28 @@ def test_map_shorter_group_geometry(qlen):
     assert out[0]["nmap"] >= 3
     for rmax in (1, 2, 3, 4, 5):
         assert _same(out[0], out[rmax]), rmax
+
+
+@pytest.mark.parametrize("qlens", [[70, 300], [319, 512, 900]])
+def test_map_streaming_band_equals_per_pair_bands(qlens):
+    """The forward and backward passes with each workgroup streaming its
+    pairs through one band give the same records, order and strings as one
+    workgroup per pair, for one-query calls (both groups) and batches."""
+    q = SynthDB.from_lengths(91, qlens)
+    d = SynthDB(92, 400, 160, 0.668, 30, 1500)
+    d.plant(q, 93, 60, 3.0, 60.0)
+    out = {}
+    for pipe in (False, True):
+        cbp = processing()
+        cbp.set_map_pipe(pipe)
+        cbp.TransferQueries(q)
+        cbp.TransferCPMData(d)
+        cbp.SetDbDetails(d.npos, d.nprof)
+        print(f"map streaming band: pipe={pipe} lengths={qlens}", flush=True)
+        one = [cbp.ProcessScoreMatrix(i, scorethld=6.0, logevthld=np.log(1e-2)) for i in range(q.nprof)]
+        batch = cbp.ProcessBatch(list(range(q.nprof)))
+        out[pipe] = (one, batch)
+        cbp.close()
+    assert sum(r["nmap"] for r in out[False][0]) >= 10
+    for a, b in zip(out[False][0] + out[False][1], out[True][0] + out[True][1]):
+        assert _same(a, b)
