@@ -458,7 +458,11 @@ __global__ __launch_bounds__(512) __attribute__((amdgpu_waves_per_eu(R == 1 ? 4 
     for (int j = (int)blockIdx.x; j < n_; j += G) {
         const int pj = m.order[o0_ + j];
         TT += c.len[m.pairs[pj].pro] + 64;
-        if (threadIdx.x == 0) m.stat[(size_t)pj * 4 + 0] = 0.0f;
+        if (threadIdx.x == 0) {
+            /* the max F, and the MAP DP's maximum key (k_map_dp_pipe) */
+            m.stat[(size_t)pj * 4 + 0] = 0.0f;
+            m.stat[(size_t)pj * 4 + 2] = 0.0f; m.stat[(size_t)pj * 4 + 3] = 0.0f;
+        }
     }
     __syncthreads();
     int r[R]; bool valid[R]; float qt[R][7];
@@ -1042,6 +1046,143 @@ __global__ __launch_bounds__(512) void k_map_dp(ScoreCtx c, MapArgs m)
     }
 }
 
+/* ---- MAP DP, one band streaming its pairs back to back ----
+ * The pair stream of k_map_fwd_pipe over the MAP DP of k_map_dp.  A pair's
+ * canonical maximum (larger value, then smaller x, then smaller y) is one
+ * 64-bit key per pair, (value bits, ~x, ~y), raised by every wave with
+ * atomicMax when its sweep of the pair ends; the last wave (the bottom
+ * stripe) ends last, and its atomicMax returns the others' maximum, so it
+ * writes the pair's record.  The keys are cleared by k_map_fwd_pipe. */
+__device__ __forceinline__ unsigned long long dp_key(float v, int x, int y)
+{
+    return v > 0.0f ? ((unsigned long long)__float_as_uint(v) << 32) |
+                          (0xFFFFFFFFu - (((uint32_t)x << 16) | (uint32_t)y))
+                    : 0ull;
+}
+
+template <int R>
+__global__ __launch_bounds__(512) void k_map_dp_pipe(ScoreCtx c, MapArgs m)
+{
+    __shared__ float hv[2][8];
+    int o0_, n_;
+    map_range(m, &o0_, &n_);
+    const int G = (int)gridDim.x;
+    if ((int)blockIdx.x >= n_) return;            /* uniform for the workgroup */
+    const int lane = threadIdx.x & 63, NW = blockDim.x >> 6;
+    const int w = __builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const int Lq = c.Lq, s = w;
+    const float minpp = m.minpp, gapreg = m.minpp * 0.5f;
+    int TT = 64 * (NW - 1);
+    for (int j = (int)blockIdx.x; j < n_; j += G) TT += c.len[m.pairs[m.order[o0_ + j]].pro] + 64;
+    int r[R];
+#pragma unroll
+    for (int k = 0; k < R; k++) r[k] = s * 64 * R + R * lane + k;
+    int jc = (int)blockIdx.x, S = 0, pi = 0, Ld = 0, nsteps = 0, e = 0, tl = 0;
+    float logZ = 0.0f;
+    const float* Fm = nullptr;
+    const float* Bm = nullptr;
+    char* bt = nullptr;
+    auto pair_at = [&](int j) {
+        pi = m.order[o0_ + j];
+        const MapPair mp = m.pairs[pi];
+        Ld = c.len[mp.pro]; nsteps = Ld + 63;
+        Fm = m.F + mp.foff; Bm = m.B + mp.foff; bt = m.btck + mp.boff;
+        logZ = m.stat[(size_t)pi * 4 + 1];
+        e = (int)rintf(0.34f * (float)(Lq > Ld ? Lq : Ld));
+        if (e > 60) e = 60;
+        tl = (Ld - 1) % 32;
+    };
+    pair_at(jc);
+    float best = 0.0f; int bx = 0, by = 0;
+    float V[R], fn_[R], bn_[R];
+    float U = 0.0f, D = 0.0f;
+    for (int T = 0; T < TT; T++) {
+        const int u = T - 64 * w;
+        if (u == S + nsteps + 1 && jc < n_) {
+            jc += G; S = u;
+            if (jc < n_) pair_at(jc);
+        }
+        const bool act = u >= S && jc < n_;
+        const int t = u - S - 1;
+        if (act && t == -1) {
+#pragma unroll
+            for (int k = 0; k < R; k++) V[k] = 0.0f;
+            U = 0.0f; D = 0.0f;
+            best = 0.0f; bx = 0; by = 0;
+#pragma unroll
+            for (int k = 0; k < R; k++) {
+                const int64_t ix = lsi<R>(s, 0, k, lane, nsteps);
+                fn_[k] = Fm[ix]; bn_[k] = Bm[ix];
+            }
+        }
+        if (act && t >= 0 && t < nsteps) {
+            const int cc = t - lane;
+            float fbc[R];
+#pragma unroll
+            for (int k = 0; k < R; k++) fbc[k] = fn_[k] + bn_[k];
+            if (t + 1 < nsteps) {
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    const int64_t ix = lsi<R>(s, t + 1, k, lane, nsteps);
+                    fn_[k] = Fm[ix]; bn_[k] = Bm[ix];
+                }
+            }
+            if (cc >= 0 && cc < Ld) {
+                float prevk = D;
+#pragma unroll
+                for (int k = 0; k < R; k++) {
+                    const int y = r[k];
+                    const int64_t ix = lsi<R>(s, t, k, lane, nsteps);
+                    const bool ok = in_valid(Lq, Ld, e, y, cc);
+                    const float fb = ok ? fbc[k] : 0.0f;
+                    const float dgv = (y > 0 && cc > 0) ? prevk : 0.0f;
+                    const bool upzero = (cc == Ld - 1 && tl != 31 && (y % 32) == 31 - tl);
+                    const float upv = (y > 0 && !upzero) ? ((k == 0) ? U : V[k - 1]) : 0.0f;
+                    const float lfv = (cc > 0) ? V[k] : 0.0f;
+                    prevk = V[k];
+                    char bc = 0;
+                    float v = dgv + mexp(fb - logZ) - minpp;
+                    if (v > 0.0f) bc = 3; else v = 0.0f;
+                    if (ok && v > 0.0f && better(v, cc, y, best, bx, by)) { best = v; bx = cc; by = y; }
+                    const float uu = upv - gapreg; if (v < uu) { v = uu; bc = 2; }
+                    const float l = lfv - gapreg; if (v < l) { v = l; bc = 1; }
+                    V[k] = v;
+                    if (y < Lq) bt[ix] = bc;
+                }
+                if (lane == 63 && w + 1 < NW) hv[T & 1][w] = V[R - 1];
+            }
+            if (t == nsteps - 1) {
+                /* the wave's sweep of the pair ends: its canonical maximum */
+                unsigned long long key = dp_key(best, bx, by);
+                for (int off = 32; off >= 1; off >>= 1) {
+                    const unsigned long long o = __shfl_xor(key, off);
+                    key = o > key ? o : key;
+                }
+                if (lane == 0) {
+                    unsigned long long* kp = (unsigned long long*)(m.stat + (size_t)pi * 4 + 2);
+                    const unsigned long long old = atomicMax(kp, key);
+                    if (w == NW - 1) {
+                        const unsigned long long k2 = old > key ? old : key;
+                        const float bv = k2 ? __uint_as_float((uint32_t)(k2 >> 32)) : 0.0f;
+                        const uint32_t xy = k2 ? 0xFFFFFFFFu - (uint32_t)k2 : 0u;
+                        int32_t* o = m.res + (size_t)pi * 8;
+                        o[5] = (int32_t)(bv > 0.0f ? xy : 0u);
+                        o[6] = __float_as_int(bv);
+                        o[7] = __float_as_int(logZ);
+                    }
+                }
+            }
+        }
+        __syncthreads();
+        if (act && t >= -1 && t + 1 < nsteps) {
+            float tp = 0.0f;
+            if (lane == 0 && w > 0 && t + 1 < Ld) tp = hv[T & 1][w - 1];
+            D = U;
+            U = wave_shr1(V[R - 1], tp);
+        }
+    }
+}
+
 /* ---- backtrace and drawing (FinalizeMAPDP_MAP), one wave per pair ----
  * c2_walk_draw over the MAP DP's codes, from the DP maximum */
 template <int R>
@@ -1121,7 +1262,9 @@ hipError_t launch_r(const ScoreCtx& c, const MapArgs& m0, int npairs, hipStream_
     } else {
         bwd(stream);
     }
-    hipLaunchKernelGGL((k_map_dp<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
+    if (pipe) hipLaunchKernelGGL((k_map_dp_pipe<R>), dim3(pipe_grid(k_map_dp_pipe<R>, m.nw, npairs)), dim3(wg), 0,
+                                 stream, c, m);
+    else hipLaunchKernelGGL((k_map_dp<R>), dim3(npairs), dim3(wg), 0, stream, c, m);
     hipLaunchKernelGGL((k_map_final<R>), dim3(npairs), dim3(64), 0, stream, c, m, npairs);
     return hipGetLastError();
 }
