@@ -607,6 +607,7 @@ int stage_map(cbp_handle* h, QWork& w, bool split)
     ma.res = w.mres.as<int32_t>(); ma.minpp = h->opt.minpp;
     ma.order = w.dmorder.as<int32_t>(); ma.R = mR; ma.ns = mns; ma.nw = mnw;
     ma.sitems = w.dmsitems.as<int4>(); ma.nsitems = (int32_t)w.msitems.size();
+    ma.pipe = h->map_pipe ? 1 : 0;
     /* strings are NUL-padded to the stride: the whole buffer is output */
     PCHK(c2_launch_zero_u32(w.malns.as<uint32_t>(), (int64_t)alnwords, w.st));
     /* every MAP pass is as long as its longest pair's column sweep; for a
